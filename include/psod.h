@@ -1,0 +1,257 @@
+/*
+ * psod.h -- C ABI of the MI355X spherical-overdensity (SOD) pass.
+ *
+ * Reference: fif092/Psod, sod/sod_profile_rdfof_parallel_hdf5.f90 ("main
+ * file").  The reference exposes Fortran external subroutines with implicit
+ * interfaces (all arguments by reference, bulk data in module globals).
+ * This header replaces them with plain C entry points: scalars by value,
+ * arrays as pointers, an int status per call.  psod_amd/fortran/psod_sod.f90
+ * binds them with ISO_C_BINDING under the reference's subroutine names.
+ *
+ * Two families of entry points:
+ *   psod_host_*   caller-owned HOST arrays (the Fortran drop-in); they upload,
+ *                 run the kernels on the context's device and download.
+ *   psod_*        DEVICE pointers already resident in HBM (hot path, bench);
+ *                 stream-ordered on the context's stream and blocking unless
+ *                 the name ends in _async.
+ *
+ * Conventions: positions are float32 xyz triplets (Fortran posf(3,N)) in box
+ * units [0,1).  Grids are x fastest: (iz*ng + iy)*ng + ix.  nx (SOD cells per
+ * side) and ng (CIC nodes per side) are powers of two; nx >= 8.  npart <
+ * 2^31.  Indices returned are 0-based except where a comment says 1-based.
+ * A context is used by one host thread at a time.
+ */
+#ifndef PSOD_H
+#define PSOD_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  The reference prints and returns (main file:1681-1693,
+ * 2072-2089); here each call returns one of these and psod_last_error()
+ * holds the message. */
+#define PSOD_OK 0
+#define PSOD_E_ARG 1        /* invalid argument                              */
+#define PSOD_E_DEVICE 2     /* HIP runtime error / no device                 */
+#define PSOD_E_CAPACITY 3   /* reference "Increase Nsel" (main file:1681)    */
+#define PSOD_E_UNCLOSED 4   /* reference "Increase selection radius" (:2072) */
+#define PSOD_E_NOMEM 5      /* device allocation failed                      */
+
+#define PSOD_MAXDELTA 4
+#define PSOD_BLOCK 8        /* cells are stored in 8^3-cell blocks            */
+
+typedef struct psod_ctx psod_ctx;
+
+/* ---- context ------------------------------------------------------------ */
+int psod_create(int device, psod_ctx **ctx);
+int psod_destroy(psod_ctx *ctx);
+const char *psod_last_error(void);
+int psod_synchronize(psod_ctx *ctx);
+/* hipStream_t of the context, for callers that order their own work on it */
+void *psod_stream(psod_ctx *ctx);
+/* bytes of device scratch the context currently holds                       */
+int64_t psod_scratch_bytes(psod_ctx *ctx);
+
+/* ---- a5: cell bucketing (main file:1561-1604) ----------------------------
+ * Replaces the quick_sort of cell keys + run-length encoding with an integer
+ * counting sort.  Cell of a particle: floor(x*nx) per axis, wrapped.  Cells
+ * are ordered by 8^3 blocks (key = block*512 + (lz*8+ly)*8+lx, blocks in
+ * (bz,by,bx) row-major order).
+ *   d_cell_start [nx^3+1]  first sorted slot of each cell key
+ *   d_perm       [npart]   original index of each sorted slot
+ *   d_rec        [4*npart] sorted records {u,v,w, idx-as-int-bits}: u,v,w is
+ *                          the position in cells relative to the origin of the
+ *                          particle's 8^3 block (exact in float32)
+ *   d_lab_sorted [npart]   labels in sorted order (NULL if d_labels is NULL)
+ * The set of particles in each cell is exact; the order inside a cell is not
+ * specified (no downstream result depends on it). */
+int psod_cell_sort(psod_ctx *ctx, int64_t npart, const float *d_pos,
+                   const int32_t *d_labels, int32_t nx, int32_t *d_cell_start,
+                   int32_t *d_perm, float *d_rec, int32_t *d_lab_sorted);
+
+/* ---- a2: CIC deposit (main file:1077-1265) ---------------------------------
+ * Vertex-centred periodic CIC of equal-mass particles onto ng^3 nodes
+ * (ng = nx or 2*nx), normalised to rho/rho_bar like the reference's
+ * mpf = 1/N followed by cube/dx^3.  shift = -xxmin*ng (mod ng) relabels the
+ * nodes as the reference's zone origin xxmin does (single zone: xxmin=-0.5).
+ * Sums are exact 64-bit fixed point, so the result does not depend on the
+ * order of the particles. */
+int psod_cic_deposit(psod_ctx *ctx, int64_t npart, int32_t nx,
+                     const int32_t *d_cell_start, const float *d_rec,
+                     int32_t ng, int32_t shift, float *d_grid);
+
+/* ---- a3 (+ FoF seeds): CIC^-1 (main file:1268-1390) -----------------------
+ * Interpolates the grid back to every particle.  Optional outputs:
+ *   d_rho_sorted [npart] in sorted order, d_rho [npart] in original order,
+ *   d_seed_key [nhalo]: for each label h >= 0 the member with the largest rho
+ *   (ties: smallest index) packed as (float bits of rho << 32 | ~index); the
+ *   caller zeroes it before the first call. */
+int psod_cic_interp(psod_ctx *ctx, int64_t npart, int32_t nx,
+                    const int32_t *d_cell_start, const float *d_rec,
+                    int32_t ng, int32_t shift, const float *d_grid,
+                    float *d_rho_sorted, float *d_rho,
+                    const int32_t *d_lab_sorted, int64_t nhalo,
+                    uint64_t *d_seed_key);
+
+/* Unpacks seed keys into particle indices (-1 for an empty label).          */
+int psod_seed_index(psod_ctx *ctx, int64_t nhalo, const uint64_t *d_seed_key,
+                    int32_t *d_seed);
+
+/* ---- a4: tri_cic (main file:998-1037) ---------------------------------------
+ * Orders particles by decreasing rho (ties: smallest index) -- the order of
+ * quick_sort on 1/rho -- and counts seeds: *npart_new = #(rho > threshold)+1
+ * (reference threshold 100, main file:1029).  d_isort is 1-based like the
+ * reference's isort. */
+int psod_density_order(psod_ctx *ctx, int64_t npart, const float *d_rho,
+                       double threshold, int32_t *d_isort,
+                       int64_t *npart_new);
+
+/* ---- a6-a12: SOD of FoF-seeded halos ------------------------------------- */
+typedef struct {
+  int32_t nx;            /* SOD cells per side; N = nx^3 gives rho_bar = 1    */
+  int32_t ndelta;        /* thresholds, first-crossing rule (main file:2046)  */
+  double delta[PSOD_MAXDELTA];
+  double delta2;         /* reference -dl2 (last-above rule); <= 0: off       */
+  int32_t nbins;         /* profile points                                    */
+  double rmin_frac;      /* first profile radius / R_delta[0]                 */
+  double rmax_frac;      /* last profile radius / R_delta[0]                  */
+  double mmin;           /* acceptance mass (reference -min)                  */
+  double mpart;          /* particle mass in rho_bar*cell^3 units (nx^3/N)    */
+  int32_t nmin_count;    /* crossing searched from this count (10, :2046)     */
+  double rsearch;        /* centre-search radius in cells (1, :1525)          */
+  double eps;            /* centre-search softening in cells (1/64, :1526)    */
+  double tol_abs;        /* boundary listing: |r - R| <= tol_abs (cells)      */
+  int32_t small_max;     /* tunable: particles per one-wave halo (0: default) */
+  int32_t flags;         /* PSOD_F_* below                                    */
+} psod_sod_params;
+
+#define PSOD_F_VELOCITY 1  /* d_vel is given: centre-of-mass velocity        */
+
+/* Per-halo outputs (device arrays, nhalo rows). */
+typedef struct {
+  int32_t *centre;       /* [nhalo]    centre particle (0-based)              */
+  int64_t *centre_id;    /* [nhalo]    its id (reference iamas)               */
+  int32_t *status;       /* [nhalo]    PSOD_H_* flags                         */
+  int32_t *w;            /* [nhalo]    reach in cells that closed the sphere  */
+  int32_t *count;        /* [nhalo*4]  particles inside R (reference masse)   */
+  double *radius;        /* [nhalo*4]  R in cells (discrete, :2091)           */
+  double *overdens;      /* [nhalo*4]  mean overdensity inside R              */
+  double *rint;          /* [nhalo*4]  log-log interpolated crossing, cells   */
+  double *mint;          /* [nhalo*4]  delta*4pi/3*rint^3, particle masses    */
+  int32_t *count2;       /* [nhalo]    reference masse2                       */
+  double *overdens2;     /* [nhalo]                                           */
+  int32_t *profile;      /* [nhalo*nbins] counts with r < r_j (:2103-2115)    */
+  double *com_pos;       /* [nhalo*3]  centre of mass of the R[0] sphere, box */
+  double *com_vel;       /* [nhalo*3]  mean velocity inside R[0]              */
+  int32_t *ngather;      /* [nhalo]    particles examined by the growth       */
+  int32_t *nboundary;    /* [nhalo]    particles with |r - R[0]| <= tol_abs   */
+} psod_halo_out;
+
+#define PSOD_H_CLOSED 1    /* every threshold crossed inside the reach        */
+#define PSOD_H_ACCEPTED 2  /* M >= mmin and centre in (0, nx] (:2118-2120)    */
+#define PSOD_H_LARGE 4     /* processed by the multi-wave path                */
+#define PSOD_H_UNCLOSED 8  /* reach would exceed nx/2                         */
+
+/* Catalogue already cell-sorted by psod_cell_sort.                          */
+typedef struct {
+  int64_t npart;
+  int32_t nx;
+  const float *pos;          /* [3*npart] original order                      */
+  const float *vel;          /* [3*npart] or NULL                             */
+  const int64_t *ids;        /* [npart] or NULL (ids = index)                 */
+  const int32_t *cell_start; /* [nx^3+1]                                      */
+  const float *rec;          /* [4*npart]                                     */
+} psod_catalogue;
+
+/* Batched split-precision pass: every halo in one stream of launches,
+ * small halos one wavefront each, large halos split over many waves.
+ * d_seed [nhalo] (from psod_seed_index), d_nmem [nhalo] member counts (a
+ * reach estimate only; may be NULL). */
+int psod_sod_fof(psod_ctx *ctx, const psod_sod_params *p,
+                 const psod_catalogue *cat, int64_t nhalo,
+                 const int32_t *d_seed, const int32_t *d_nmem,
+                 const psod_halo_out *out);
+
+/* Plain method: all-FP64 arithmetic, one launch sequence per halo.  Same
+ * outputs; the comparator of the batched pass. */
+int psod_sod_fof_plain(psod_ctx *ctx, const psod_sod_params *p,
+                       const psod_catalogue *cat, int64_t nhalo,
+                       const int32_t *d_seed, const int32_t *d_nmem,
+                       const psod_halo_out *out);
+
+/* Member counts per label (labels < 0 ignored).                             */
+int psod_label_counts(psod_ctx *ctx, int64_t npart, const int32_t *d_labels,
+                      int64_t nhalo, int32_t *d_nmem);
+
+/* ---- synthetic catalogue (no reference counterpart: benchmark input) ------
+ * Seeded NFW clumps on a uniform background; see or_gen_* in the oracle for
+ * the same specification on the CPU. */
+typedef struct {
+  int64_t npart, nclump;
+  double nlo, nhi, alpha, clo, chi, trunc;
+  uint64_t seed;
+  int64_t ncentral;
+  double ccentral;
+  double sig_clump, sig_bulk, sig_bg;
+} psod_gen_params;
+
+/* Host part: clump table (nclump[+1] rows).  Returns the clumped particles. */
+int64_t psod_gen_clumps(const psod_gen_params *g, int64_t *size,
+                        int64_t *offset, double *centre, double *conc,
+                        double *r200, double *bulk);
+/* Device part: particles [0, npart) from a clump table (host arrays).       */
+int psod_generate(psod_ctx *ctx, const psod_gen_params *g,
+                  const int64_t *offset, const double *centre,
+                  const double *conc, const double *r200, const double *bulk,
+                  float *d_pos, float *d_vel, int32_t *d_labels,
+                  int64_t *d_ids);
+
+/* ---- host-pointer entry points (Fortran drop-in) ---------------------------
+ * Each replaces one reference subroutine for a single zone covering the box.
+ */
+
+/* `cic` (main file:1077-1109): equal masses (mass may be NULL or all equal),
+ * zone origin xxmin with xxmin*ng integral.  grid [ng^3] float32. */
+int psod_host_cic(psod_ctx *ctx, int64_t npart, const float *pos,
+                  const float *mass, double xxmin, int32_t ng, float *grid);
+
+/* `cicm1` (main file:1268-1285): rho [npart] at each particle. */
+int psod_host_cicm1(psod_ctx *ctx, int64_t npart, const float *pos,
+                    double xxmin, int32_t ng, const float *grid, float *rho);
+
+/* `tri_cic` (main file:998-1006): isort [npart] 1-based, decreasing rho. */
+int psod_host_tri_cic(psod_ctx *ctx, int64_t npart, const float *rho,
+                      double threshold, int32_t *isort, int64_t *npart_new);
+
+/* `spherover` (main file:1396-1400), greedy seeds-with-exclusion semantics.
+ * Positions are UNscaled box units; nx, ncut, seuil, seuil2, mmin, nsel as the
+ * reference flags; nbins/rmin_frac/rmax_frac set the profile (60, 0.01, 1).
+ * Outputs: one row per processed seed (accepted or not), *nrec rows:
+ *   rec_i64 [maxrec*6]: ipart(1-based), seed, centre, iamas, masse, masse2
+ *   rec_f64 [maxrec*8]: amass, xc, yc, zc (cells), rayon, overdens,
+ *                       overdens2, accepted(0/1)
+ *   rad_bins [maxrec*nbins], mass_bins [maxrec*nbins]
+ *   alive [npart] final exclusion flags; structure [npart] accepted iamas.
+ * Returns PSOD_E_CAPACITY / PSOD_E_UNCLOSED like the reference's zone abort. */
+typedef struct {
+  int32_t nx, ncut;
+  double seuil, seuil2, mmin;
+  int64_t nsel;
+  int32_t nbins;
+  double rmin_frac, rmax_frac;
+} psod_greedy_params;
+
+int psod_host_spherover(psod_ctx *ctx, int64_t npart, const float *pos,
+                        const int64_t *ids, const int32_t *isort,
+                        int64_t npart_new, const psod_greedy_params *p,
+                        int64_t maxrec, int64_t *nrec, int64_t *rec_i64,
+                        double *rec_f64, double *rad_bins, int32_t *mass_bins,
+                        uint8_t *alive, int64_t *structure);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

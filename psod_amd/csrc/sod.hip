@@ -1,0 +1,1295 @@
+// sod.hip -- a6-a12: batched split-precision SOD of FoF-seeded halos.
+//
+// Each halo (FoF label) is one work item:
+//   seed   = its member with the largest CIC density (cic.hip);
+//   centre = argmax over the particles of the 3^3 cells around the seed cell
+//            of S(c) = sum over particles within 1 cell of 1/sqrt(d2+eps^2)
+//            (main file:1795-1875);
+//   growth = first k >= 10 with O(k) <= Delta for each threshold, R = r_{k-1}
+//            (main file:2040-2100), last k above Delta2 (:2062-2069), the
+//            log profile (:2103-2115), centre of mass inside R.
+// Unlike the reference, which gathers a fixed (2*ncut+1)^3 cube around the
+// seed, the growth here gathers every particle within a reach W of the
+// centre and doubles W until every requested radius fits inside it.
+//
+// Positions are split: records hold float32 offsets inside 8^3-cell blocks,
+// so relative coordinates are (whole cells) + (offset difference) and stay
+// exact to float32 resolution of the in-cell offsets.  S is accumulated in
+// 2^-24 fixed point, so it does not depend on the order of the neighbours.
+//
+// Small halos (<= CAP5 particles around the seed, reach <= WMAX_S) run one
+// per wavefront from a persistent queue; everything they touch stays in LDS
+// and their distances are sorted there.  Large halos split the centre search
+// into (halo, candidate cell, 64 candidates) tasks and grow with radial
+// histograms (bins uniform in r^3) whose ambiguous bins are resolved exactly
+// from the collected particles, one workgroup per halo.
+#include "sod_common.h"
+
+#include <cstring>
+
+int psod_fill_i32(psod_ctx *c, int32_t *p, int64_t n, int32_t v);
+int psod_fill_u64(psod_ctx *c, uint64_t *p, int64_t n, uint64_t v);
+
+constexpr int CAP5 = 1024;   // small path: particles in the 5^3 cells
+constexpr int CAPG = 2048;   // small path: particles in the growth cube
+constexpr int WMAX_S = 4;    // small path: largest reach
+constexpr int NCUBE_S = (2 * WMAX_S + 1) * (2 * WMAX_S + 1) * (2 * WMAX_S + 1);
+
+__device__ inline int32_t ldcg(const int32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---------------------------------------------------------------------------
+// host helpers
+// ---------------------------------------------------------------------------
+SodCfg make_cfg(const psod_sod_params *p, int64_t npart) {
+  SodCfg c;
+  memset(&c, 0, sizeof(c));
+  c.nx = p->nx;
+  c.ndelta = p->ndelta;
+  c.nbins = p->nbins;
+  c.nmin_count = p->nmin_count > 1 ? p->nmin_count : 2;
+  c.flags = p->flags;
+  for (int i = 0; i < PSOD_MAXDELTA; i++)
+    c.delta[i] = i < p->ndelta ? p->delta[i] : 0.0;
+  c.delta2 = p->delta2;
+  c.lr0 = log(p->rmin_frac);
+  c.lr1 = log(p->rmax_frac);
+  c.rmin_frac = p->rmin_frac;
+  c.rmax_frac = p->rmax_frac;
+  c.mmin = p->mmin;
+  c.mpart = p->mpart;
+  c.rs2 = p->rsearch * p->rsearch;
+  c.eps2 = p->eps * p->eps;
+  c.rs2f = (float)c.rs2;
+  c.eps2f = (float)c.eps2;
+  c.tol_abs = p->tol_abs;
+  c.nbar = (double)npart / ((double)p->nx * p->nx * p->nx);
+  return c;
+}
+
+int check_sod_args(const psod_sod_params *p, const psod_catalogue *cat,
+                   int64_t nhalo, const int32_t *seed,
+                   const psod_halo_out *out) {
+  if (!p || !cat || !out || (nhalo > 0 && !seed))
+    return psod_set_error(PSOD_E_ARG, "sod: null argument");
+  if (p->nx != cat->nx || p->nx < 8 || p->nx > 1024 || (p->nx & (p->nx - 1)))
+    return psod_set_error(PSOD_E_ARG, "sod: nx=%d (catalogue %d)", p->nx,
+                          cat->nx);
+  if (p->ndelta < 1 || p->ndelta > PSOD_MAXDELTA)
+    return psod_set_error(PSOD_E_ARG, "sod: ndelta=%d", p->ndelta);
+  for (int i = 0; i < p->ndelta; i++)
+    if (!(p->delta[i] > 0.0))
+      return psod_set_error(PSOD_E_ARG, "sod: delta[%d]=%g", i, p->delta[i]);
+  if (p->ndelta > 1)
+    for (int i = 1; i < p->ndelta; i++)
+      if (p->delta[i] < p->delta[0])
+        return psod_set_error(PSOD_E_ARG,
+                              "sod: delta[0] must be the smallest threshold");
+  if (p->nbins < 2 || p->nbins > 256 || !(p->rmin_frac > 0.0) ||
+      !(p->rmax_frac >= p->rmin_frac) || !(p->rmax_frac >= 1.0))
+    return psod_set_error(PSOD_E_ARG, "sod: profile nbins=%d [%g, %g]",
+                          p->nbins, p->rmin_frac, p->rmax_frac);
+  if (!(p->mpart > 0.0) || p->nmin_count < 2)
+    return psod_set_error(PSOD_E_ARG, "sod: mpart=%g nmin_count=%d", p->mpart,
+                          p->nmin_count);
+  if (!cat->pos || !cat->cell_start || !cat->rec || cat->npart <= 0)
+    return psod_set_error(PSOD_E_ARG, "sod: catalogue is incomplete");
+  if ((p->flags & PSOD_F_VELOCITY) && !cat->vel)
+    return psod_set_error(PSOD_E_ARG, "sod: velocity requested without vel");
+  return PSOD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// plan: seed cell, particles around it, reach estimate, small/large
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_plan(SodCfg c, int64_t nhalo, const int32_t *__restrict__ seed,
+       const int32_t *__restrict__ nmem, const float *__restrict__ pos,
+       const int32_t *__restrict__ cell_start, HaloPlan *__restrict__ plan,
+       int32_t *__restrict__ small, int32_t *__restrict__ large,
+       int32_t *__restrict__ counters, psod_halo_out out) {
+  const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= nhalo) return;
+  HaloPlan pl;
+  pl.seed = seed[h];
+  pl.nmem = nmem ? nmem[h] : 0;
+  pl.centre = -1;
+  pl.ntask = 0;
+  pl.task0 = 0;
+  pl.pad = 0;
+  if (pl.seed < 0) {
+    pl.klass = -1;
+    pl.w0 = 0;
+    pl.n5 = 0;
+    pl.cs[0] = pl.cs[1] = pl.cs[2] = 0;
+    plan[h] = pl;
+    HaloResult r;
+    zero_result(r);
+    store_result(out, h, c.nbins, r);
+    for (int j = 0; j < c.nbins; j++) out.profile[h * c.nbins + j] = 0;
+    return;
+  }
+  const CellRef sr = ref_of_particle(pos, pl.seed, c.nx);
+  int32_t n5 = 0;
+  for (int q = 0; q < 125; q++) {
+    int di, dj, dk;
+    walk_off(q, 2, &di, &dj, &dk);
+    int32_t l[3];
+    const int32_t key = key_at(c.nx, sr.c, di, dj, dk, l);
+    n5 += cell_start[key + 1] - cell_start[key];
+  }
+  for (int d = 0; d < 3; d++) pl.cs[d] = sr.c[d];
+  pl.n5 = n5;
+  pl.w0 = reach_estimate(c, pl.nmem);
+  pl.klass = (n5 <= CAP5 && pl.w0 <= WMAX_S) ? 0 : 1;
+  plan[h] = pl;
+  if (pl.klass == 0)
+    small[atomicAdd(&counters[0], 1)] = (int32_t)h;
+  else
+    large[atomicAdd(&counters[1], 1)] = (int32_t)h;
+}
+
+// ---------------------------------------------------------------------------
+// wave helpers
+// ---------------------------------------------------------------------------
+// exclusive prefix of v over the 64 lanes plus the wave total
+__device__ inline int32_t wave_excl(int32_t v, int32_t *tot) {
+  const int32_t inc = wave_incl_scan(v);
+  *tot = __shfl(inc, 63, 64);
+  return inc - v;
+}
+
+__device__ inline bool better(uint64_t a, uint64_t ka, uint64_t b,
+                              uint64_t kb) {
+  return a > b || (a == b && ka < kb);
+}
+
+__device__ inline void wave_argmax(uint64_t &acc, uint64_t &key) {
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint64_t a2 = __shfl_xor(acc, o, 64);
+    const uint64_t k2 = __shfl_xor(key, o, 64);
+    if (better(a2, k2, acc, key)) { acc = a2; key = k2; }
+  }
+}
+
+// bitonic sort of n (power of two) u64 keys in LDS by one workgroup
+__device__ inline void lds_bitonic(uint64_t *a, int n) {
+  for (int k = 2; k <= n; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const int p = i ^ j;
+        if (p > i) {
+          const uint64_t x = a[i], y = a[p];
+          const bool up = (i & k) == 0;
+          if ((x > y) == up) { a[i] = y; a[p] = x; }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+__device__ inline int next_pow2(int n) {
+  int p = 1;
+  while (p < n) p <<= 1;
+  return p;
+}
+
+__device__ inline uint64_t d2key(float d2, int32_t idx) {
+  return ((uint64_t)__float_as_uint(d2) << 32) | (uint32_t)idx;
+}
+__device__ inline float key_d2(uint64_t k) {
+  return __uint_as_float((uint32_t)(k >> 32));
+}
+__device__ inline int32_t key_idx(uint64_t k) {
+  return (int32_t)(uint32_t)(k & 0xFFFFFFFFull);
+}
+
+// ---------------------------------------------------------------------------
+// small halos: one wavefront (= one workgroup of 64) per halo
+// ---------------------------------------------------------------------------
+struct SmallA {
+  float4 nb[CAP5];        // relative to the seed cell origin, w = index bits
+  int8_t rank[CAP5];      // candidate cell rank 0..26, -1 otherwise
+  int16_t cidx[CAP5];     // compacted candidate list
+  int32_t pre[128];       // prefix over the 125 cells
+  int32_t st[128];        // first slot of each of the 125 cells
+};
+struct SmallB {
+  uint64_t key[CAPG];
+  int32_t pre[NCUBE_S + 1];
+  int32_t st[NCUBE_S];
+};
+union SmallLds {
+  SmallA a;
+  SmallB b;
+};
+
+__device__ inline int bsearch_last_le(const int32_t *pre, int n, int32_t j) {
+  int lo = 0, hi = n;  // last c in [0, n) with pre[c] <= j
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (pre[mid] <= j) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+__device__ void small_growth_and_output(const SodCfg &c, SmallLds &S,
+                                        const int32_t *cell_start,
+                                        const float4 *rec, const float *pos,
+                                        const float *vel, const int64_t *ids,
+                                        int32_t centre, int32_t w0,
+                                        psod_halo_out out, int64_t h,
+                                        HaloPlan *plan, int32_t *large,
+                                        int32_t *counters);
+
+__global__ void __launch_bounds__(64)
+k_sod_small(SodCfg c, const int32_t *__restrict__ cell_start,
+            const float4 *__restrict__ rec, const float *__restrict__ pos,
+            const float *__restrict__ vel, const int64_t *__restrict__ ids,
+            HaloPlan *__restrict__ plan, const int32_t *__restrict__ small,
+            int32_t *__restrict__ large, int32_t *__restrict__ counters,
+            psod_halo_out out) {
+  __shared__ SmallLds S;
+  __shared__ int32_t s_item;
+  const int l = threadIdx.x;
+  const int32_t nsmall = counters[0];
+  for (;;) {
+    if (l == 0) s_item = atomicAdd(&counters[2], 1);
+    __syncthreads();
+    const int32_t it = s_item;
+    __syncthreads();
+    if (it >= nsmall) break;
+    const int64_t h = small[it];
+    HaloPlan pl = plan[h];
+    // ---- the 125 cells around the seed cell ---------------------------
+    int32_t cnt0 = 0, cnt1 = 0, st0 = 0, st1 = 0;
+    {
+      int di, dj, dk;
+      int32_t lc[3];
+      walk_off(l, 2, &di, &dj, &dk);
+      int32_t key = key_at(c.nx, pl.cs, di, dj, dk, lc);
+      st0 = cell_start[key];
+      cnt0 = cell_start[key + 1] - st0;
+      if (l + 64 < 125) {
+        walk_off(l + 64, 2, &di, &dj, &dk);
+        key = key_at(c.nx, pl.cs, di, dj, dk, lc);
+        st1 = cell_start[key];
+        cnt1 = cell_start[key + 1] - st1;
+      }
+    }
+    int32_t t0, t1;
+    const int32_t e0 = wave_excl(cnt0, &t0);
+    const int32_t e1 = wave_excl(cnt1, &t1);
+    S.a.pre[l] = e0;
+    S.a.st[l] = st0;
+    if (l + 64 < 125) {
+      S.a.pre[l + 64] = t0 + e1;
+      S.a.st[l + 64] = st1;
+    }
+    const int32_t n5 = t0 + t1;
+    if (l == 0) S.a.pre[125] = n5;
+    __syncthreads();
+    // n5 <= CAP5 was checked by the plan
+    for (int32_t j = l; j < n5; j += 64) {
+      const int q = bsearch_last_le(S.a.pre, 125, j);
+      const float4 r = rec[S.a.st[q] + (j - S.a.pre[q])];
+      int di, dj, dk;
+      walk_off(q, 2, &di, &dj, &dk);
+      int32_t lc[3];
+      key_at(c.nx, pl.cs, di, dj, dk, lc);
+      float4 v;
+      v.x = rel1(r.x, lc[0], di, 0.0f);
+      v.y = rel1(r.y, lc[1], dj, 0.0f);
+      v.z = rel1(r.z, lc[2], dk, 0.0f);
+      v.w = r.w;
+      S.a.nb[j] = v;
+      const bool inner = di >= -1 && di <= 1 && dj >= -1 && dj <= 1 &&
+                         dk >= -1 && dk <= 1;
+      S.a.rank[j] = inner ? (int8_t)((di + 1) * 9 + (dj + 1) * 3 + (dk + 1))
+                          : (int8_t)-1;
+    }
+    __syncthreads();
+    // compact the candidates
+    int32_t ncand = 0;
+    for (int32_t base = 0; base < n5; base += 64) {
+      const int32_t j = base + l;
+      const bool isc = j < n5 && S.a.rank[j] >= 0;
+      const uint64_t m = ballot64(isc);
+      const uint64_t lower = (l == 0) ? 0ull : (~0ull >> (64 - l));
+      if (isc) S.a.cidx[ncand + __popcll(m & lower)] = (int16_t)j;
+      ncand += __popcll(m);
+    }
+    __syncthreads();
+    // ---- centre search -------------------------------------------------
+    uint64_t bacc = 0, bkey = ~0ull;
+    for (int32_t ci = l; ci < ncand; ci += 64) {
+      const int j = S.a.cidx[ci];
+      const float4 p = S.a.nb[j];
+      uint64_t acc = 0;
+      for (int32_t t = 0; t < n5; t++) {
+        const float4 q = S.a.nb[t];
+        const float dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
+        const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        if (d2 <= c.rs2f) acc += sterm_fix(d2, c.eps2f);
+      }
+      const uint64_t key = ((uint64_t)(uint32_t)S.a.rank[j] << 32) |
+                           (uint32_t)__float_as_int(p.w);
+      if (better(acc, key, bacc, bkey)) { bacc = acc; bkey = key; }
+    }
+    wave_argmax(bacc, bkey);
+    const int32_t centre = key_idx(bkey);
+    __syncthreads();
+    small_growth_and_output(c, S, cell_start, rec, pos, vel, ids, centre,
+                            pl.w0, out, h, plan, large, counters);
+    __syncthreads();
+  }
+}
+
+// growth of a small halo around `centre`; defers to the large path when the
+// cube does not fit
+__device__ void small_growth_and_output(const SodCfg &c, SmallLds &S,
+                                        const int32_t *cell_start,
+                                        const float4 *rec, const float *pos,
+                                        const float *vel, const int64_t *ids,
+                                        int32_t centre, int32_t w0,
+                                        psod_halo_out out, int64_t h,
+                                        HaloPlan *plan, int32_t *large,
+                                        int32_t *counters) {
+  const int l = threadIdx.x;
+  const CellRef cr = ref_of_particle(pos, centre, c.nx);
+  int32_t W = w0;
+  HaloResult res;
+  zero_result(res);
+  res.centre = centre;
+  res.centre_id = ids ? ids[centre] : (int64_t)centre;
+  for (;;) {
+    if (W > WMAX_S || 2 * W + 1 > c.nx) {
+      if (l == 0) {
+        plan[h].centre = centre;
+        plan[h].w0 = W;
+        large[atomicAdd(&counters[1], 1)] = (int32_t)h;
+      }
+      return;
+    }
+    const int side = 2 * W + 1, ncell = side * side * side;
+    // ranges of the cube cells
+    int32_t carry = 0;
+    for (int base = 0; base < ncell; base += 64) {
+      const int q = base + l;
+      int32_t cnt = 0, st = 0;
+      if (q < ncell) {
+        int di, dj, dk;
+        walk_off(q, W, &di, &dj, &dk);
+        int32_t lc[3];
+        const int32_t key = key_at(c.nx, cr.c, di, dj, dk, lc);
+        st = cell_start[key];
+        cnt = cell_start[key + 1] - st;
+      }
+      int32_t tot;
+      const int32_t ex = wave_excl(cnt, &tot);
+      if (q < ncell) {
+        S.b.pre[q] = carry + ex;
+        S.b.st[q] = st;
+      }
+      carry += tot;
+    }
+    const int32_t ncube = carry;
+    if (l == 0) S.b.pre[ncell] = ncube;
+    __syncthreads();
+    if (ncube > CAPG) {
+      __syncthreads();
+      W = WMAX_S + 1;  // defer
+      continue;
+    }
+    for (int32_t j = l; j < ncube; j += 64) {
+      const int q = bsearch_last_le(S.b.pre, ncell, j);
+      const float4 r = rec[S.b.st[q] + (j - S.b.pre[q])];
+      int di, dj, dk;
+      walk_off(q, W, &di, &dj, &dk);
+      int32_t lc[3];
+      key_at(c.nx, cr.c, di, dj, dk, lc);
+      const float dx = rel1(r.x, lc[0], di, cr.f[0]);
+      const float dy = rel1(r.y, lc[1], dj, cr.f[1]);
+      const float dz = rel1(r.z, lc[2], dk, cr.f[2]);
+      const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+      S.b.key[j] = d2key(d2, __float_as_int(r.w));
+    }
+    const int npad = next_pow2(ncube < 64 ? 64 : ncube);
+    for (int32_t j = ncube + l; j < npad; j += 64) S.b.key[j] = ~0ull;
+    __syncthreads();
+    lds_bitonic(S.b.key, npad);
+    // ---- first crossing of every threshold -----------------------------
+    bool complete = true;
+    int64_t kst[PSOD_MAXDELTA];
+    for (int di = 0; di < c.ndelta; di++) {
+      kst[di] = -1;
+      for (int32_t base = c.nmin_count; base <= ncube && kst[di] < 0;
+           base += 64) {
+        const int64_t k = base + l;
+        bool hit = false, over = false;
+        if (k <= ncube) {
+          const double r = (double)sqrtf(key_d2(S.b.key[k - 1]));
+          over = r >= (double)W;
+          hit = over || od_k(k, c.mpart, r) <= c.delta[di];
+        }
+        const uint64_t m = ballot64(hit);
+        if (m) {
+          const int first = __ffsll((unsigned long long)m) - 1;
+          const bool fo = __shfl(over ? 1 : 0, first, 64);
+          kst[di] = fo ? -2 : base + first;
+        }
+      }
+      if (kst[di] < 0) complete = false;
+    }
+    double R0 = 0.0;
+    if (complete) {
+      R0 = (double)sqrtf(key_d2(S.b.key[kst[0] - 2]));
+      if (c.rmax_frac * R0 > (double)W) complete = false;
+    }
+    if (!complete) {
+      int32_t wn = 2 * W;
+      if (kst[0] > 0) {
+        const int32_t need =
+            (int32_t)ceil(c.rmax_frac *
+                          (double)sqrtf(key_d2(S.b.key[kst[0] - 2]))) + 1;
+        if (need > wn) wn = need;
+      }
+      W = wn;
+      __syncthreads();
+      continue;
+    }
+    res.w = W;
+    res.ngather = ncube;
+    double xcs[3];
+    centre_scaled(pos, centre, c.nx, xcs);
+    for (int di = 0; di < c.ndelta; di++) {
+      const int64_t ks = kst[di];
+      finish_delta(c, di, ks,
+                   ref_dist(pos, key_idx(S.b.key[ks - 2]), xcs, c.nx),
+                   ref_dist(pos, key_idx(S.b.key[ks - 1]), xcs, c.nx), res);
+    }
+    R0 = res.radius[0];
+    // ---- second threshold: last k in [2, k*0] above delta2 ----------------
+    if (c.delta2 > 0.0) {
+      int64_t best = 0;
+      for (int32_t base = 2; base <= kst[0]; base += 64) {
+        const int64_t k = base + l;
+        bool hit = false;
+        if (k <= kst[0]) {
+          const double r = (double)sqrtf(key_d2(S.b.key[k - 1]));
+          hit = od_k(k, c.mpart, r) > c.delta2;
+        }
+        const uint64_t m = ballot64(hit);
+        if (m) best = base + 63 - __clzll((unsigned long long)m);
+      }
+      res.count2 = (int32_t)best;
+      if (best > 0) {
+        const double r = ref_dist(pos, key_idx(S.b.key[best - 1]), xcs, c.nx);
+        res.overdens2 = r > 1e-30 ? od_k(best, c.mpart, r) : 0.0;
+      }
+    }
+    // ---- profile --------------------------------------------------------
+    for (int j = l; j < c.nbins; j += 64) {
+      const double rj = prof_radius(c, R0, j);
+      int lo = 0, hi = ncube;  // first index with r >= rj
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((double)sqrtf(key_d2(S.b.key[mid])) < rj) lo = mid + 1;
+        else hi = mid;
+      }
+      out.profile[h * c.nbins + j] = lo;
+    }
+    // ---- boundary particles, centre of mass of the count[0] closest -------
+    int32_t nbd = 0;
+    double sp[3] = {0, 0, 0}, sv[3] = {0, 0, 0};
+    const int32_t nin = res.count[0];
+    const bool flagv = (c.flags & PSOD_F_VELOCITY) && vel;
+    const double xc[3] = {(double)cr.c[0] + (double)cr.f[0],
+                          (double)cr.c[1] + (double)cr.f[1],
+                          (double)cr.c[2] + (double)cr.f[2]};
+    for (int32_t k = l; k < ncube; k += 64) {
+      const uint64_t key = S.b.key[k];
+      const double r = (double)sqrtf(key_d2(key));
+      if (fabs(r - R0) <= c.tol_abs) nbd++;
+      if (k < nin) {
+        const int32_t pj = key_idx(key);
+        for (int d = 0; d < 3; d++) {
+          int32_t cj;
+          float fj;
+          split_coord(pos[3 * (int64_t)pj + d], c.nx, &cj, &fj);
+          double dd = ((double)cj + (double)fj) - xc[d];
+          if (dd > 0.5 * c.nx) dd -= c.nx;
+          if (dd < -0.5 * c.nx) dd += c.nx;
+          sp[d] += dd;
+          if (flagv) sv[d] += (double)vel[3 * (int64_t)pj + d];
+        }
+      }
+    }
+    nbd = wave_sum(nbd);
+    for (int d = 0; d < 3; d++) {
+      sp[d] = wave_sum(sp[d]);
+      sv[d] = wave_sum(sv[d]);
+      double cp = (xc[d] + sp[d] / (double)nin) / (double)c.nx;
+      cp -= floor(cp);
+      res.com_pos[d] = cp;
+      res.com_vel[d] = sv[d] / (double)nin;
+    }
+    res.nboundary = nbd;
+    bool acc = (double)res.count[0] * c.mpart >= c.mmin;
+    for (int d = 0; d < 3; d++) acc = acc && xc[d] > 0.0 && xc[d] <= c.nx;
+    res.status = PSOD_H_CLOSED | (acc ? PSOD_H_ACCEPTED : 0);
+    if (l == 0) store_result(out, h, c.nbins, res);
+    return;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// large halos: centre search tasks
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_task_count(SodCfg c, const int32_t *__restrict__ cell_start,
+             HaloPlan *__restrict__ plan, const int32_t *__restrict__ large,
+             int32_t nlarge, int32_t *__restrict__ ntask) {
+  const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nlarge) return;
+  const HaloPlan pl = plan[large[e]];
+  int32_t nt = 0;
+  if (pl.centre < 0) {
+    for (int q = 0; q < 27; q++) {
+      int di, dj, dk;
+      walk_off(q, 1, &di, &dj, &dk);
+      int32_t lc[3];
+      const int32_t key = key_at(c.nx, pl.cs, di, dj, dk, lc);
+      const int32_t n = cell_start[key + 1] - cell_start[key];
+      nt += (n + 63) / 64;
+    }
+  }
+  ntask[e] = nt;
+}
+
+__global__ void __launch_bounds__(256)
+k_task_emit(SodCfg c, const int32_t *__restrict__ cell_start,
+            HaloPlan *__restrict__ plan, const int32_t *__restrict__ large,
+            int32_t nlarge, const int32_t *__restrict__ ntask,
+            const int32_t *__restrict__ task0, int4 *__restrict__ tasks) {
+  const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nlarge) return;
+  const int32_t h = large[e];
+  plan[h].ntask = ntask[e];
+  plan[h].task0 = task0[e];
+  if (ntask[e] == 0) return;
+  const HaloPlan pl = plan[h];
+  int32_t t = task0[e];
+  for (int q = 0; q < 27; q++) {
+    int di, dj, dk;
+    walk_off(q, 1, &di, &dj, &dk);
+    int32_t lc[3];
+    const int32_t key = key_at(c.nx, pl.cs, di, dj, dk, lc);
+    const int32_t a = cell_start[key], b = cell_start[key + 1];
+    for (int32_t s = a; s < b; s += 64)
+      tasks[t++] = make_int4(e, q, s, min(b, s + 64) - s);
+  }
+}
+
+// one wave per task: up to 64 candidates of one cell against the 27 cells
+// around that cell (every neighbour within 1 cell lies there)
+__global__ void __launch_bounds__(256)
+k_centre_task(SodCfg c, const int32_t *__restrict__ cell_start,
+              const float4 *__restrict__ rec, const HaloPlan *__restrict__ plan,
+              const int32_t *__restrict__ large, const int4 *__restrict__ tasks,
+              const int32_t *__restrict__ ntask_total,
+              uint64_t *__restrict__ taskres) {
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int32_t nt = *ntask_total;
+  for (int32_t t = blockIdx.x * 4 + w; t < nt; t += gridDim.x * 4) {
+    const int4 tk = tasks[t];
+    const HaloPlan pl = plan[large[tk.x]];
+    int ci, cj, ck;
+    walk_off(tk.y, 1, &ci, &cj, &ck);
+    int32_t lc[3];
+    key_at(c.nx, pl.cs, ci, cj, ck, lc);
+    float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+    const bool have = l < tk.w;
+    if (have) {
+      const float4 r = rec[tk.z + l];
+      p.x = rel1(r.x, lc[0], ci, 0.0f);
+      p.y = rel1(r.y, lc[1], cj, 0.0f);
+      p.z = rel1(r.z, lc[2], ck, 0.0f);
+      p.w = r.w;
+    }
+    uint64_t acc = 0;
+    for (int q = 0; q < 27; q++) {
+      int ei, ej, ek;
+      walk_off(q, 1, &ei, &ej, &ek);
+      const int di = ci + ei, dj = cj + ej, dk = ck + ek;
+      int32_t ln[3];
+      const int32_t key = key_at(c.nx, pl.cs, di, dj, dk, ln);
+      const int32_t a = cell_start[key], b = cell_start[key + 1];
+      for (int32_t s = a; s < b; s += 64) {
+        const int32_t m = min(64, b - s);
+        float vx = 0.f, vy = 0.f, vz = 0.f;
+        if (l < m) {
+          const float4 r = rec[s + l];
+          vx = rel1(r.x, ln[0], di, 0.0f);
+          vy = rel1(r.y, ln[1], dj, 0.0f);
+          vz = rel1(r.z, ln[2], dk, 0.0f);
+        }
+        for (int32_t u = 0; u < m; u++) {
+          const float qx = __int_as_float(
+              __builtin_amdgcn_readlane(__float_as_int(vx), u));
+          const float qy = __int_as_float(
+              __builtin_amdgcn_readlane(__float_as_int(vy), u));
+          const float qz = __int_as_float(
+              __builtin_amdgcn_readlane(__float_as_int(vz), u));
+          const float dx = qx - p.x, dy = qy - p.y, dz = qz - p.z;
+          const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+          if (d2 <= c.rs2f) acc += sterm_fix(d2, c.eps2f);
+        }
+      }
+    }
+    uint64_t key = have ? (((uint64_t)(uint32_t)tk.y << 32) |
+                           (uint32_t)__float_as_int(p.w))
+                        : ~0ull;
+    if (!have) acc = 0;
+    wave_argmax(acc, key);
+    if (l == 0) {
+      taskres[2 * (int64_t)t] = acc;
+      taskres[2 * (int64_t)t + 1] = key;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_centre_reduce(HaloPlan *__restrict__ plan, const int32_t *__restrict__ large,
+                int32_t nlarge, const uint64_t *__restrict__ taskres) {
+  const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nlarge) return;
+  const int32_t h = large[e];
+  HaloPlan pl = plan[h];
+  if (pl.centre >= 0 || pl.ntask == 0) return;
+  uint64_t ba = 0, bk = ~0ull;
+  for (int32_t t = pl.task0; t < pl.task0 + pl.ntask; t++) {
+    const uint64_t a = taskres[2 * (int64_t)t], k = taskres[2 * (int64_t)t + 1];
+    if (better(a, k, ba, bk)) { ba = a; bk = k; }
+  }
+  plan[h].centre = key_idx(bk);
+}
+
+// ---------------------------------------------------------------------------
+// large halos: growth with radial histograms, one workgroup per halo
+// ---------------------------------------------------------------------------
+constexpr int GL_T = 256;
+constexpr int SORT_CAP = 4096;
+constexpr int MAXRANGE = PSOD_MAXDELTA + 1;
+
+struct GrowShared {
+  uint64_t skey[SORT_CAP];
+  int32_t pre[GL_T + 1];
+  int32_t st[GL_T];
+  int32_t red_i[GL_T / 64];
+  int64_t red_l[GL_T / 64];
+  double red_d[GL_T / 64];
+  double prad[257];
+  int32_t pcnt[257];
+  int32_t item;
+  int32_t fill[MAXRANGE];
+  int64_t off[MAXRANGE];
+};
+
+__device__ inline int32_t block_min_i32(int32_t v, GrowShared &G) {
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) G.red_i[threadIdx.x >> 6] = v;
+  __syncthreads();
+  int32_t r = G.red_i[0];
+  for (int i = 1; i < GL_T / 64; i++) r = min(r, G.red_i[i]);
+  __syncthreads();
+  return r;
+}
+__device__ inline int32_t block_max_i32(int32_t v, GrowShared &G) {
+  return -block_min_i32(-v, G);
+}
+__device__ inline int64_t block_sum_i64(int64_t v, GrowShared &G) {
+  v = wave_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) G.red_l[threadIdx.x >> 6] = v;
+  __syncthreads();
+  int64_t r = 0;
+  for (int i = 0; i < GL_T / 64; i++) r += G.red_l[i];
+  __syncthreads();
+  return r;
+}
+__device__ inline double block_sum_f64(double v, GrowShared &G) {
+  v = wave_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) G.red_d[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = 0;
+  for (int i = 0; i < GL_T / 64; i++) r += G.red_d[i];
+  __syncthreads();
+  return r;
+}
+
+// Calls f(d2, idx, rec, di, dj, dk, lc) for every particle of the cells at
+// offsets [-W, W]^3 from the reference cell.
+template <typename F>
+__device__ inline void cube_walk(const SodCfg &c, GrowShared &G,
+                                 const int32_t *cell_start, const float4 *rec,
+                                 const CellRef &cr, int W, F &&f) {
+  const int side = 2 * W + 1;
+  const int32_t ncell = side * side * side;
+  for (int32_t base = 0; base < ncell; base += GL_T) {
+    const int32_t q = base + threadIdx.x;
+    int32_t cnt = 0, st = 0;
+    if (q < ncell) {
+      int di, dj, dk;
+      walk_off(q, W, &di, &dj, &dk);
+      int32_t lc[3];
+      const int32_t key = key_at(c.nx, cr.c, di, dj, dk, lc);
+      st = cell_start[key];
+      cnt = cell_start[key + 1] - st;
+    }
+    // block exclusive scan of cnt
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    int32_t tot;
+    const int32_t ex = wave_excl(cnt, &tot);
+    if (l == 0) G.red_i[w] = tot;
+    __syncthreads();
+    int32_t off = 0, all = 0;
+    for (int i = 0; i < GL_T / 64; i++) {
+      if (i < w) off += G.red_i[i];
+      all += G.red_i[i];
+    }
+    G.pre[threadIdx.x] = off + ex;
+    G.st[threadIdx.x] = st;
+    if (threadIdx.x == 0) G.pre[GL_T] = all;
+    __syncthreads();
+    const int nq = min(GL_T, ncell - base);
+    for (int32_t j = threadIdx.x; j < all; j += GL_T) {
+      const int qq = bsearch_last_le(G.pre, nq, j);
+      const float4 r = rec[G.st[qq] + (j - G.pre[qq])];
+      int di, dj, dk;
+      walk_off(base + qq, W, &di, &dj, &dk);
+      int32_t lc[3];
+      key_at(c.nx, cr.c, di, dj, dk, lc);
+      const float dx = rel1(r.x, lc[0], di, cr.f[0]);
+      const float dy = rel1(r.y, lc[1], dj, cr.f[1]);
+      const float dz = rel1(r.z, lc[2], dk, cr.f[2]);
+      const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+      f(d2, __float_as_int(r.w), r, di, dj, dk, lc);
+    }
+    __syncthreads();
+  }
+}
+
+// radial bin of d2 for nb bins uniform in r^3 over [0, W^3)
+__device__ inline int32_t rbin(float d2, float scale, int32_t nb) {
+  const float v = d2 * sqrtf(d2) * scale;
+  const int32_t b = (int32_t)v;
+  return b < nb ? b : nb - 1;
+}
+
+// bins whose edges may be compared with exact radii: widen by 1e-5
+__device__ inline double edge_lo(int32_t b, int32_t nb, double W) {
+  return W * cbrt((double)b / (double)nb) * (1.0 - 1e-5) - 1e-6;
+}
+__device__ inline double edge_hi(int32_t b, int32_t nb, double W) {
+  return W * cbrt((double)(b + 1) / (double)nb) * (1.0 + 1e-5) + 1e-6;
+}
+
+// Pool slots reserved for n collected keys: n itself when the keys are
+// sorted in LDS, the next power of two otherwise (padding for the bitonic
+// network on global memory).
+__device__ inline int64_t pool_slots(int64_t n) {
+  if (n <= SORT_CAP) return n;
+  int64_t p = 1;
+  while (p < n) p <<= 1;
+  return p;
+}
+
+// sorts the n keys at pool[0..n) ascending; pool holds pool_slots(n) slots
+__device__ void block_sort_pool(uint64_t *pool, int32_t n, GrowShared &G) {
+  if (n <= 1) return;
+  if (n <= SORT_CAP) {
+    const int np = next_pow2(n);
+    for (int i = threadIdx.x; i < np; i += GL_T)
+      G.skey[i] = i < n ? pool[i] : ~0ull;
+    __syncthreads();
+    lds_bitonic(G.skey, np);
+    for (int i = threadIdx.x; i < n; i += GL_T) pool[i] = G.skey[i];
+    __threadfence_block();
+    __syncthreads();
+    return;
+  }
+  const int np = next_pow2(n);
+  for (int i = n + threadIdx.x; i < np; i += GL_T) pool[i] = ~0ull;
+  __threadfence_block();
+  __syncthreads();
+  for (int k = 2; k <= np; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < np; i += GL_T) {
+        const int p = i ^ j;
+        if (p > i) {
+          const uint64_t x = pool[i], y = pool[p];
+          const bool up = (i & k) == 0;
+          if ((x > y) == up) { pool[i] = y; pool[p] = x; }
+        }
+      }
+      __threadfence_block();
+      __syncthreads();
+    }
+  }
+}
+
+__global__ void __launch_bounds__(GL_T)
+k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
+             const float4 *__restrict__ rec, const float *__restrict__ pos,
+             const float *__restrict__ vel, const int64_t *__restrict__ ids,
+             HaloPlan *__restrict__ plan, const int32_t *__restrict__ large,
+             int32_t *__restrict__ counters, int32_t *__restrict__ hist_all,
+             const int64_t *__restrict__ hist_off, uint64_t *__restrict__ pool,
+             int64_t pool_cap, unsigned long long *__restrict__ pool_ctr,
+             psod_halo_out out) {
+  __shared__ GrowShared G;
+  const int32_t nlarge = counters[1];
+  for (;;) {
+    if (threadIdx.x == 0) G.item = atomicAdd(&counters[3], 1);
+    __syncthreads();
+    const int32_t e = G.item;
+    __syncthreads();
+    if (e >= nlarge) break;
+    const int64_t h = large[e];
+    const HaloPlan pl = plan[h];
+    const int32_t centre = pl.centre;
+    const CellRef cr = ref_of_particle(pos, centre, c.nx);
+    int32_t *hist = hist_all + hist_off[e];
+    const int32_t nb = (int32_t)(hist_off[e + 1] - hist_off[e]) - 1;
+    HaloResult res;
+    zero_result(res);
+    res.centre = centre;
+    res.centre_id = ids ? ids[centre] : (int64_t)centre;
+    int32_t W = pl.w0 < 2 ? 2 : pl.w0;
+    bool failed = false;
+    double xcs[3];
+    centre_scaled(pos, centre, c.nx, xcs);
+    for (;;) {
+      if (2 * W + 1 > c.nx) {
+        res.status = PSOD_H_UNCLOSED | PSOD_H_LARGE;
+        res.w = W;
+        break;
+      }
+      const float W2 = (float)W * (float)W;
+      const float scale = (float)nb / ((float)W * (float)W * (float)W);
+      for (int32_t b = threadIdx.x; b <= nb; b += GL_T) hist[b] = 0;
+      __threadfence_block();
+      __syncthreads();
+      // ---- pass 1: histogram -------------------------------------------
+      int64_t ng = 0;
+      cube_walk(c, G, cell_start, rec, cr, W,
+                [&](float d2, int32_t, const float4 &, int, int, int,
+                    const int32_t *) {
+                  ng++;
+                  if (d2 < W2) atomicAdd(&hist[rbin(d2, scale, nb)], 1);
+                });
+      ng = block_sum_i64(ng, G);
+      __threadfence();
+      __syncthreads();
+      // exclusive scan in place: hist[b] = #(bin < b), hist[nb] = total
+      {
+        int32_t carry = 0;
+        for (int32_t base = 0; base <= nb; base += GL_T) {
+          const int32_t b = base + threadIdx.x;
+          const int32_t v = b < nb ? ldcg(&hist[b]) : 0;
+          const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+          int32_t tot;
+          const int32_t ex = wave_excl(v, &tot);
+          if (l == 0) G.red_i[w] = tot;
+          __syncthreads();
+          int32_t off = 0, all = 0;
+          for (int i = 0; i < GL_T / 64; i++) {
+            if (i < w) off += G.red_i[i];
+            all += G.red_i[i];
+          }
+          if (b <= nb) hist[b] = carry + off + ex;
+          carry += all;
+          __syncthreads();
+        }
+      }
+      __threadfence_block();
+      __syncthreads();
+      const int32_t *C = hist;
+      const double Wd = (double)W;
+      // ---- crossing bands of every threshold -----------------------------
+      int32_t b1[PSOD_MAXDELTA], b2[PSOD_MAXDELTA], bp[PSOD_MAXDELTA];
+      bool ok = true;
+      for (int di = 0; di < c.ndelta && ok; di++) {
+        int32_t first_any = INT32_MAX, first_def = INT32_MAX;
+        for (int32_t b = threadIdx.x; b < nb; b += GL_T) {
+          const int32_t cb = ldcg(&C[b]), ce = ldcg(&C[b + 1]);
+          const int64_t kmin = max((int64_t)cb + 1, (int64_t)c.nmin_count);
+          if (kmin > ce) continue;
+          const double f = rad_for(kmin, c.mpart, c.delta[di]);
+          if (f <= edge_lo(b, nb, Wd)) {
+            first_def = min(first_def, b);
+            first_any = min(first_any, b);
+          } else if (f < edge_hi(b, nb, Wd)) {
+            first_any = min(first_any, b);
+          }
+        }
+        b1[di] = block_min_i32(first_any, G);
+        b2[di] = block_min_i32(first_def, G);
+        if (b2[di] == INT32_MAX) { ok = false; break; }
+        int32_t prev = -1;
+        for (int32_t b = threadIdx.x; b < b1[di]; b += GL_T)
+          if (ldcg(&C[b + 1]) > ldcg(&C[b])) prev = max(prev, b);
+        bp[di] = block_max_i32(prev, G);
+        if (bp[di] < 0) bp[di] = b1[di];
+      }
+      if (!ok) {
+        W = 2 * W;
+        __syncthreads();
+        continue;
+      }
+      // ---- merge the bands into disjoint ranges --------------------------
+      int32_t rs[MAXRANGE], re[MAXRANGE], nr = 0;
+      for (int di = 0; di < c.ndelta; di++) {
+        int32_t s = bp[di], t = b2[di];
+        bool merged = false;
+        for (int r = 0; r < nr; r++)
+          if (!(t < rs[r] - 1 || s > re[r] + 1)) {
+            rs[r] = min(rs[r], s);
+            re[r] = max(re[r], t);
+            merged = true;
+          }
+        if (!merged) { rs[nr] = s; re[nr] = t; nr++; }
+      }
+      int64_t need = 0;
+      for (int r = 0; r < nr; r++)
+        need += pool_slots(ldcg(&C[re[r] + 1]) - ldcg(&C[rs[r]]));
+      if (threadIdx.x == 0) {
+        const unsigned long long o = atomicAdd(pool_ctr, (unsigned long long)need);
+        int64_t oo = (int64_t)o;
+        for (int r = 0; r < nr; r++) {
+          G.off[r] = oo;
+          G.fill[r] = 0;
+          oo += pool_slots(ldcg(&C[re[r] + 1]) - ldcg(&C[rs[r]]));
+        }
+        if ((int64_t)o + need > pool_cap) G.off[0] = -1;
+      }
+      __syncthreads();
+      if (G.off[0] < 0) { failed = true; break; }
+      // ---- pass 2: collect the bands --------------------------------------
+      cube_walk(c, G, cell_start, rec, cr, W,
+                [&](float d2, int32_t idx, const float4 &, int, int, int,
+                    const int32_t *) {
+                  if (!(d2 < W2)) return;
+                  const int32_t b = rbin(d2, scale, nb);
+                  for (int r = 0; r < nr; r++)
+                    if (b >= rs[r] && b <= re[r]) {
+                      const int32_t s = atomicAdd(&G.fill[r], 1);
+                      pool[G.off[r] + s] = d2key(d2, idx);
+                    }
+                });
+      __threadfence_block();
+      __syncthreads();
+      for (int r = 0; r < nr; r++)
+        block_sort_pool(pool + G.off[r], ldcg(&C[re[r] + 1]) - ldcg(&C[rs[r]]),
+                        G);
+      // ---- exact first crossing ------------------------------------------
+      int64_t kst[PSOD_MAXDELTA];
+      uint64_t keyR = 0;
+      for (int di = 0; di < c.ndelta; di++) {
+        int r = 0;
+        for (int q = 0; q < nr; q++)
+          if (bp[di] >= rs[q] && b2[di] <= re[q]) r = q;
+        const int64_t k0 = ldcg(&C[rs[r]]);  // rank before the range
+        const int64_t klo = max((int64_t)ldcg(&C[b1[di]]) + 1,
+                                (int64_t)c.nmin_count);
+        const int64_t khi = ldcg(&C[b2[di] + 1]);
+        int32_t firstk = INT32_MAX;
+        for (int64_t k = klo + threadIdx.x; k <= khi; k += GL_T) {
+          const double rk = (double)sqrtf(key_d2(pool[G.off[r] + (k - k0 - 1)]));
+          if (od_k(k, c.mpart, rk) <= c.delta[di]) firstk = min(firstk, (int32_t)k);
+        }
+        firstk = block_min_i32(firstk, G);
+        kst[di] = firstk == INT32_MAX ? -1 : firstk;
+        if (kst[di] < 0) { ok = false; break; }
+        const double r1 = ref_dist(
+            pos, key_idx(pool[G.off[r] + (kst[di] - 1 - k0 - 1)]), xcs, c.nx);
+        const double r2 = ref_dist(
+            pos, key_idx(pool[G.off[r] + (kst[di] - k0 - 1)]), xcs, c.nx);
+        finish_delta(c, di, kst[di], r1, r2, res);
+        if (di == 0) keyR = pool[G.off[r] + (kst[di] - 1 - k0 - 1)];
+      }
+      if (!ok) {
+        W = 2 * W;
+        __syncthreads();
+        continue;
+      }
+      const double R0 = res.radius[0];
+      if (c.rmax_frac * R0 > Wd) {
+        const int32_t need_w = (int32_t)ceil(c.rmax_frac * R0) + 1;
+        W = max(2 * W, need_w);
+        __syncthreads();
+        continue;
+      }
+      // ---- second-threshold band (last k above delta2) ---------------------
+      int32_t b3 = -1, b4 = -1;
+      const int64_t k0s = kst[0];
+      if (c.delta2 > 0.0) {
+        int32_t last_any = -1, last_def = -1;
+        for (int32_t b = threadIdx.x; b < nb; b += GL_T) {
+          const int32_t cb = ldcg(&C[b]), ce = ldcg(&C[b + 1]);
+          const int64_t klo = max((int64_t)cb + 1, (int64_t)2);
+          const int64_t kmax = min((int64_t)ce, k0s);
+          if (kmax < klo) continue;
+          const double g = rad_for(kmax, c.mpart, c.delta2);
+          if (g >= edge_hi(b, nb, Wd)) {
+            last_def = max(last_def, b);
+            last_any = max(last_any, b);
+          } else if (g > edge_lo(b, nb, Wd)) {
+            last_any = max(last_any, b);
+          }
+        }
+        b3 = block_max_i32(last_any, G);
+        b4 = block_max_i32(last_def, G);
+      }
+      const bool need2 = b3 >= 0;
+      const int32_t s2 = b4 >= 0 ? b4 : 0;
+      if (need2) {
+        const int64_t n2 = pool_slots(ldcg(&C[b3 + 1]) - ldcg(&C[s2]));
+        if (threadIdx.x == 0) {
+          const unsigned long long o =
+              atomicAdd(pool_ctr, (unsigned long long)n2);
+          G.off[MAXRANGE - 1] = (int64_t)o + n2 > pool_cap ? -1 : (int64_t)o;
+          G.fill[MAXRANGE - 1] = 0;
+        }
+      }
+      // profile radii
+      for (int j = threadIdx.x; j < c.nbins; j += GL_T) {
+        G.prad[j] = prof_radius(c, R0, j);
+        G.pcnt[j] = 0;
+      }
+      __syncthreads();
+      if (need2 && G.off[MAXRANGE - 1] < 0) { failed = true; break; }
+      // ---- pass 3: profile, centre of mass, boundary, band of delta2 -------
+      const double rlast = G.prad[c.nbins - 1];
+      double sp[3] = {0, 0, 0}, sv[3] = {0, 0, 0};
+      int64_t nbd = 0;
+      const int flagv = (c.flags & PSOD_F_VELOCITY) && vel;
+      cube_walk(c, G, cell_start, rec, cr, W,
+                [&](float d2, int32_t idx, const float4 &r, int di, int dj,
+                    int dk, const int32_t *lc) {
+                  if (!(d2 < W2)) return;
+                  const double rr = (double)sqrtf(d2);
+                  if (rr < rlast) {
+                    int lo = 0, hi = c.nbins - 1;  // first j with rr < prad[j]
+                    while (lo < hi) {
+                      const int mid = (lo + hi) >> 1;
+                      if (rr < G.prad[mid]) hi = mid; else lo = mid + 1;
+                    }
+                    atomicAdd(&G.pcnt[lo], 1);
+                  }
+                  if (fabs(rr - R0) <= c.tol_abs) nbd++;
+                  if (d2key(d2, idx) <= keyR) {
+                    sp[0] += rel1d(r.x, lc[0], di, (double)cr.f[0]);
+                    sp[1] += rel1d(r.y, lc[1], dj, (double)cr.f[1]);
+                    sp[2] += rel1d(r.z, lc[2], dk, (double)cr.f[2]);
+                    if (flagv)
+                      for (int d = 0; d < 3; d++)
+                        sv[d] += (double)vel[3 * (int64_t)idx + d];
+                  }
+                  if (need2) {
+                    const int32_t b = rbin(d2, scale, nb);
+                    if (b >= s2 && b <= b3) {
+                      const int32_t s = atomicAdd(&G.fill[MAXRANGE - 1], 1);
+                      pool[G.off[MAXRANGE - 1] + s] = d2key(d2, idx);
+                    }
+                  }
+                });
+      nbd = block_sum_i64(nbd, G);
+      const int32_t nin = res.count[0];
+      const double xc[3] = {(double)cr.c[0] + (double)cr.f[0],
+                            (double)cr.c[1] + (double)cr.f[1],
+                            (double)cr.c[2] + (double)cr.f[2]};
+      for (int d = 0; d < 3; d++) {
+        const double s = block_sum_f64(sp[d], G);
+        const double v = block_sum_f64(sv[d], G);
+        double cp = (xc[d] + s / (double)nin) / (double)c.nx;
+        cp -= floor(cp);
+        res.com_pos[d] = cp;
+        res.com_vel[d] = v / (double)nin;
+      }
+      // cumulative profile
+      if (threadIdx.x == 0) {
+        int32_t s = 0;
+        for (int j = 0; j < c.nbins; j++) {
+          s += G.pcnt[j];
+          out.profile[h * c.nbins + j] = s;
+        }
+      }
+      // ---- delta2 --------------------------------------------------------
+      if (c.delta2 > 0.0) {
+        int64_t best = 0;
+        if (need2) {
+          __threadfence_block();
+          __syncthreads();
+          const int32_t n2 = ldcg(&C[b3 + 1]) - ldcg(&C[s2]);
+          uint64_t *p2 = pool + G.off[MAXRANGE - 1];
+          block_sort_pool(p2, n2, G);
+          const int64_t kb = ldcg(&C[s2]);
+          int32_t bestk = -1;
+          for (int32_t t = threadIdx.x; t < n2; t += GL_T) {
+            const int64_t k = kb + t + 1;
+            if (k < 2 || k > k0s) continue;
+            const double rk = (double)sqrtf(key_d2(p2[t]));
+            if (od_k(k, c.mpart, rk) > c.delta2) bestk = max(bestk, (int32_t)k);
+          }
+          bestk = block_max_i32(bestk, G);
+          if (bestk > 0) best = bestk;
+        }
+        res.count2 = (int32_t)best;
+        if (best > 0) {
+          // rank `best` lies in the collected band of delta2
+          const int64_t kb = ldcg(&C[s2]);
+          const double rb = ref_dist(
+              pos, key_idx(pool[G.off[MAXRANGE - 1] + (best - kb - 1)]), xcs,
+              c.nx);
+          res.overdens2 = rb > 1e-30 ? od_k(best, c.mpart, rb) : 0.0;
+        }
+      }
+      res.w = W;
+      res.ngather = (int32_t)ng;
+      res.nboundary = (int32_t)nbd;
+      bool acc = (double)res.count[0] * c.mpart >= c.mmin;
+      for (int d = 0; d < 3; d++) acc = acc && xc[d] > 0.0 && xc[d] <= c.nx;
+      res.status = PSOD_H_CLOSED | PSOD_H_LARGE | (acc ? PSOD_H_ACCEPTED : 0);
+      break;
+    }
+    if (failed) res.status = PSOD_H_LARGE | PSOD_H_UNCLOSED;
+    if (threadIdx.x == 0) store_result(out, h, c.nbins, res);
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host orchestration
+// ---------------------------------------------------------------------------
+__global__ void k_hist_sizes(SodCfg c, const HaloPlan *__restrict__ plan,
+                             const int32_t *__restrict__ large, int32_t nlarge,
+                             int32_t *__restrict__ nb) {
+  const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nlarge) return;
+  const HaloPlan pl = plan[large[e]];
+  const double w = (double)(pl.w0 < 2 ? 2 : pl.w0);
+  const double est = 2.0 * ((double)pl.nmem + 4.19 * w * w * w * c.nbar);
+  int32_t n = 4096;
+  while (n < est && n < (1 << 22)) n <<= 1;
+  nb[e] = n + 1;
+}
+
+__global__ void k_i32_to_i64_scan(const int32_t *__restrict__ ex,
+                                  const int32_t *__restrict__ v, int32_t n,
+                                  int64_t *__restrict__ out) {
+  // ex is an int32 exclusive scan whose total fits in int32
+  const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) out[e] = ex[e];
+  if (e == n - 1) out[n] = (int64_t)ex[e] + v[e];
+}
+
+extern "C" int psod_sod_fof(psod_ctx *ctx, const psod_sod_params *p,
+                            const psod_catalogue *cat, int64_t nhalo,
+                            const int32_t *d_seed, const int32_t *d_nmem,
+                            const psod_halo_out *out) {
+  PSOD_TRY(psod_use_device(ctx));
+  PSOD_TRY(check_sod_args(p, cat, nhalo, d_seed, out));
+  if (nhalo <= 0) return PSOD_OK;
+  if (!out->profile)
+    return psod_set_error(PSOD_E_ARG, "sod: profile output is required");
+  const SodCfg c = make_cfg(p, cat->npart);
+  hipStream_t st = ctx->stream;
+  // plan, lists and counters
+  HaloPlan *plan = (HaloPlan *)psod_scratch(ctx, S_PLAN,
+                                            (size_t)nhalo * sizeof(HaloPlan));
+  int32_t *lists = (int32_t *)psod_scratch(ctx, S_TASK,
+                                           (size_t)(2 * nhalo + 64) * 4);
+  if (!plan || !lists) return PSOD_E_NOMEM;
+  int32_t *small = lists, *large = lists + nhalo, *counters = lists + 2 * nhalo;
+  PSOD_TRY(psod_fill_i32(ctx, counters, 16, 0));
+  k_plan<<<grid_for(nhalo, 256), 256, 0, st>>>(
+      c, nhalo, d_seed, d_nmem, cat->pos, cat->cell_start, plan, small, large,
+      counters, *out);
+  PSOD_LAUNCH_CHECK();
+  // small halos, persistent: a few workgroups per CU
+  k_sod_small<<<(unsigned)ctx->num_cus * 8, 64, 0, st>>>(
+      c, cat->cell_start, (const float4 *)cat->rec, cat->pos, cat->vel,
+      cat->ids, plan, small, large, counters, *out);
+  PSOD_LAUNCH_CHECK();
+  int32_t hc[4];
+  PSOD_HIP(hipMemcpyAsync(hc, counters, 16, hipMemcpyDeviceToHost, st));
+  PSOD_HIP(hipStreamSynchronize(st));
+  const int32_t nlarge = hc[1];
+  if (nlarge == 0) return PSOD_OK;
+  // centre-search tasks of the large halos
+  int32_t *ntask = (int32_t *)psod_scratch(ctx, S_MISC,
+                                           (size_t)(3 * nlarge + 64) * 4);
+  if (!ntask) return PSOD_E_NOMEM;
+  int32_t *task0 = ntask + nlarge + 16;
+  int32_t *ntot = task0 + nlarge + 16;
+  k_task_count<<<grid_for(nlarge, 256), 256, 0, st>>>(c, cat->cell_start,
+                                                       plan, large, nlarge,
+                                                       ntask);
+  PSOD_LAUNCH_CHECK();
+  PSOD_TRY(psod_scan_i32(ctx, ntask, task0, nlarge, ntot));
+  int32_t nt = 0;
+  PSOD_HIP(hipMemcpyAsync(&nt, ntot, 4, hipMemcpyDeviceToHost, st));
+  PSOD_HIP(hipStreamSynchronize(st));
+  if (nt > 0) {
+    int4 *tasks = (int4 *)psod_scratch(ctx, S_TASKRES,
+                                       (size_t)nt * (sizeof(int4) + 16));
+    if (!tasks) return PSOD_E_NOMEM;
+    uint64_t *tres = (uint64_t *)(tasks + nt);
+    k_task_emit<<<grid_for(nlarge, 256), 256, 0, st>>>(
+        c, cat->cell_start, plan, large, nlarge, ntask, task0, tasks);
+    PSOD_LAUNCH_CHECK();
+    k_centre_task<<<grid_for((int64_t)nt, 4, (int64_t)ctx->num_cus * 16), 256,
+                    0, st>>>(c, cat->cell_start, (const float4 *)cat->rec,
+                             plan, large, tasks, ntot, tres);
+    PSOD_LAUNCH_CHECK();
+    k_centre_reduce<<<grid_for(nlarge, 256), 256, 0, st>>>(plan, large,
+                                                           nlarge, tres);
+    PSOD_LAUNCH_CHECK();
+  }
+  // histogram slices
+  int32_t *nbv = (int32_t *)psod_scratch(ctx, S_PLAIN2,
+                                         (size_t)(2 * nlarge + 64) * 4 +
+                                             (size_t)(nlarge + 8) * 8);
+  if (!nbv) return PSOD_E_NOMEM;
+  int32_t *nbx = nbv + nlarge + 16;
+  int64_t *hoff = (int64_t *)(nbx + nlarge + 16);
+  k_hist_sizes<<<grid_for(nlarge, 256), 256, 0, st>>>(c, plan, large, nlarge,
+                                                      nbv);
+  PSOD_LAUNCH_CHECK();
+  PSOD_TRY(psod_scan_i32(ctx, nbv, nbx, nlarge, nullptr));
+  k_i32_to_i64_scan<<<grid_for(nlarge, 256), 256, 0, st>>>(nbx, nbv, nlarge,
+                                                           hoff);
+  PSOD_LAUNCH_CHECK();
+  int64_t htot = 0;
+  PSOD_HIP(hipMemcpyAsync(&htot, hoff + nlarge, 8, hipMemcpyDeviceToHost, st));
+  PSOD_HIP(hipStreamSynchronize(st));
+  int32_t *hist = (int32_t *)psod_scratch(ctx, S_BINS, (size_t)htot * 4 + 64);
+  const int64_t pool_cap = (int64_t)1 << 25;
+  uint64_t *pool = (uint64_t *)psod_scratch(ctx, S_COLLECT,
+                                            (size_t)pool_cap * 8 + 64);
+  if (!hist || !pool) return PSOD_E_NOMEM;
+  unsigned long long *pctr = (unsigned long long *)(pool + pool_cap);
+  PSOD_TRY(psod_fill_u64(ctx, (uint64_t *)pctr, 1, 0));
+  k_grow_large<<<(unsigned)min(nlarge, ctx->num_cus * 2), GL_T, 0, st>>>(
+      c, cat->cell_start, (const float4 *)cat->rec, cat->pos, cat->vel,
+      cat->ids, plan, large, counters, hist, hoff, pool, pool_cap, pctr, *out);
+  PSOD_LAUNCH_CHECK();
+  PSOD_HIP(hipStreamSynchronize(st));
+  return PSOD_OK;
+}
