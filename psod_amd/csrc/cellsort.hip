@@ -86,6 +86,114 @@ k_cell_scatter(const float *__restrict__ pos, const int32_t *__restrict__ lab,
 
 int psod_fill_i32(psod_ctx *c, int32_t *p, int64_t n, int32_t v);
 
+// ---------------------------------------------------------------------------
+// Dense cells (>= 64 particles) are reordered by the Morton key of the
+// particle's 1/8-cell sub-cell, so that any 64 consecutive particles of a
+// dense cell occupy a compact box.  The order inside a cell carries no
+// meaning elsewhere; the centre search uses the compact chunks to skip
+// neighbour chunks beyond its 1-cell reach.
+// ---------------------------------------------------------------------------
+constexpr int DENSE = 64;
+constexpr int SUB_T = 256, SUB_V = 16, SUB_REG = SUB_T * SUB_V;
+constexpr int64_t SUB_TMP = 1 << 18;  // records per workgroup slice
+
+__device__ inline uint32_t spread3(uint32_t v) {
+  return (v & 1) | ((v & 2) << 2) | ((v & 4) << 4);
+}
+__device__ inline uint32_t morton9(const float4 &r) {
+  const uint32_t qx = (uint32_t)((r.x - floorf(r.x)) * 8.0f) & 7;
+  const uint32_t qy = (uint32_t)((r.y - floorf(r.y)) * 8.0f) & 7;
+  const uint32_t qz = (uint32_t)((r.z - floorf(r.z)) * 8.0f) & 7;
+  return spread3(qx) | (spread3(qy) << 1) | (spread3(qz) << 2);
+}
+
+__global__ void k_dense_cells(const int32_t *__restrict__ cs, int64_t ncell,
+                              int32_t *__restrict__ list,
+                              int32_t *__restrict__ cnt) {
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < ncell;
+       c += (int64_t)gridDim.x * blockDim.x)
+    if (cs[c + 1] - cs[c] >= DENSE) list[atomicAdd(cnt, 1)] = (int32_t)c;
+}
+
+__global__ void __launch_bounds__(SUB_T)
+k_subsort(const int32_t *__restrict__ cs, const int32_t *__restrict__ list,
+          const int32_t *__restrict__ nlist, int32_t *__restrict__ queue,
+          float4 *__restrict__ rec, int32_t *__restrict__ perm,
+          int32_t *__restrict__ lab, float4 *__restrict__ trec,
+          int32_t *__restrict__ tperm, int32_t *__restrict__ tlab) {
+  __shared__ int32_t hist[512];
+  __shared__ int32_t item;
+  const int32_t nl = *nlist;
+  float4 *tr = trec + (int64_t)blockIdx.x * SUB_TMP;
+  int32_t *tp = tperm + (int64_t)blockIdx.x * SUB_TMP;
+  int32_t *tl = tlab + (int64_t)blockIdx.x * SUB_TMP;
+  for (;;) {
+    if (threadIdx.x == 0) item = atomicAdd(queue, 1);
+    for (int i = threadIdx.x; i < 512; i += SUB_T) hist[i] = 0;
+    __syncthreads();
+    const int32_t it = item;
+    if (it >= nl) break;
+    const int32_t c = list[it];
+    const int32_t a = cs[c], n = cs[c + 1] - a;
+    if (n > SUB_TMP) {  // would not fit the slice: keep the scatter order
+      __syncthreads();
+      continue;
+    }
+    for (int32_t t = threadIdx.x; t < n; t += SUB_T)
+      atomicAdd(&hist[morton9(rec[a + t])], 1);
+    __syncthreads();
+    if (threadIdx.x < 64) {  // exclusive scan of 512 counters by one wave
+      int32_t v[8], s = 0;
+      for (int k = 0; k < 8; k++) { v[k] = hist[threadIdx.x * 8 + k]; s += v[k]; }
+      const int32_t inc = wave_incl_scan(s);
+      int32_t e = inc - s;
+      for (int k = 0; k < 8; k++) { hist[threadIdx.x * 8 + k] = e; e += v[k]; }
+    }
+    __syncthreads();
+    if (n <= SUB_REG) {
+      // in place: everything is read into registers before any write
+      float4 rr[SUB_V];
+      int32_t pp[SUB_V], ll[SUB_V], dd[SUB_V];
+#pragma unroll
+      for (int k = 0; k < SUB_V; k++) {
+        const int32_t t = threadIdx.x + k * SUB_T;
+        if (t < n) {
+          rr[k] = rec[a + t];
+          pp[k] = perm ? perm[a + t] : 0;
+          ll[k] = lab ? lab[a + t] : 0;
+          dd[k] = atomicAdd(&hist[morton9(rr[k])], 1);
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < SUB_V; k++) {
+        const int32_t t = threadIdx.x + k * SUB_T;
+        if (t < n) {
+          rec[a + dd[k]] = rr[k];
+          if (perm) perm[a + dd[k]] = pp[k];
+          if (lab) lab[a + dd[k]] = ll[k];
+        }
+      }
+    } else {
+      for (int32_t t = threadIdx.x; t < n; t += SUB_T) {
+        const float4 r = rec[a + t];
+        const int32_t d = atomicAdd(&hist[morton9(r)], 1);
+        tr[d] = r;
+        if (perm) tp[d] = perm[a + t];
+        if (lab) tl[d] = lab[a + t];
+      }
+      __threadfence_block();
+      __syncthreads();
+      for (int32_t t = threadIdx.x; t < n; t += SUB_T) {
+        rec[a + t] = tr[t];
+        if (perm) perm[a + t] = tp[t];
+        if (lab) lab[a + t] = tl[t];
+      }
+    }
+    __syncthreads();
+  }
+}
+
 extern "C" int psod_cell_sort(psod_ctx *ctx, int64_t npart, const float *d_pos,
                               const int32_t *d_labels, int32_t nx,
                               int32_t *d_cell_start, int32_t *d_perm,
@@ -113,6 +221,26 @@ extern "C" int psod_cell_sort(psod_ctx *ctx, int64_t npart, const float *d_pos,
     k_cell_scatter<<<g, 256, 0, ctx->stream>>>(
         d_pos, d_labels, npart, nx, d_cell_start, count, d_perm,
         (float4 *)d_rec, d_lab_sorted);
+    PSOD_LAUNCH_CHECK();
+  }
+  if (npart > 0 && d_rec) {
+    // Morton order of the 1/8-cell sub-cells inside dense cells; the fill
+    // counters are free again and hold the dense-cell list
+    int32_t *list = count;
+    int32_t *ctl = (int32_t *)psod_scratch(ctx, S_MISC, 64);
+    if (!ctl) return PSOD_E_NOMEM;
+    PSOD_TRY(psod_fill_i32(ctx, ctl, 2, 0));
+    k_dense_cells<<<grid_for(ncell, 256, (int64_t)ctx->num_cus * 64), 256, 0,
+                    ctx->stream>>>(d_cell_start, ncell, list, ctl);
+    PSOD_LAUNCH_CHECK();
+    const unsigned nsub = (unsigned)ctx->num_cus;
+    char *tmp = (char *)psod_scratch(ctx, S_KEYS,
+                                     (size_t)nsub * SUB_TMP * 24);
+    if (!tmp) return PSOD_E_NOMEM;
+    k_subsort<<<nsub, SUB_T, 0, ctx->stream>>>(
+        d_cell_start, list, ctl, ctl + 1, (float4 *)d_rec, d_perm,
+        d_lab_sorted, (float4 *)tmp, (int32_t *)(tmp + (size_t)nsub * SUB_TMP * 16),
+        (int32_t *)(tmp + (size_t)nsub * SUB_TMP * 20));
     PSOD_LAUNCH_CHECK();
   }
   PSOD_HIP(hipStreamSynchronize(ctx->stream));

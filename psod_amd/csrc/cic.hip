@@ -167,34 +167,56 @@ k_cic_interp(const int32_t *__restrict__ cell_start,
   }
   __syncthreads();
   const int32_t s0 = cell_start[blk * BS3], s1 = cell_start[blk * BS3 + BS3];
-  for (int32_t k = s0 + threadIdx.x; k < s1; k += 256) {
-    const float4 r = rec[k];
-    const float qx = (float)R * r.x, qy = (float)R * r.y, qz = (float)R * r.z;
-    const float fx0 = floorf(qx), fy0 = floorf(qy), fz0 = floorf(qz);
-    const int ix = (int)fx0, iy = (int)fy0, iz = (int)fz0;
-    const float fx = qx - fx0, fy = qy - fy0, fz = qz - fz0;
-    const double wx[2] = {1.0 - (double)fx, (double)fx};
-    const double wy[2] = {1.0 - (double)fy, (double)fy};
-    const double wz[2] = {1.0 - (double)fz, (double)fz};
-    double acc = 0.0;
+  // uniform trip count so that every lane of a wave takes part in the
+  // per-label reduction of the seed keys
+  for (int32_t base = s0; base < s1; base += 256) {
+    const int32_t k = base + threadIdx.x;
+    const bool ok = k < s1;
+    float rf = 0.0f;
+    int32_t idx = 0;
+    if (ok) {
+      const float4 r = rec[k];
+      const float qx = (float)R * r.x, qy = (float)R * r.y, qz = (float)R * r.z;
+      const float fx0 = floorf(qx), fy0 = floorf(qy), fz0 = floorf(qz);
+      const int ix = (int)fx0, iy = (int)fy0, iz = (int)fz0;
+      const float fx = qx - fx0, fy = qy - fy0, fz = qz - fz0;
+      const double wx[2] = {1.0 - (double)fx, (double)fx};
+      const double wy[2] = {1.0 - (double)fy, (double)fy};
+      const double wz[2] = {1.0 - (double)fz, (double)fz};
+      double acc = 0.0;
 #pragma unroll
-    for (int c = 0; c < 8; c++) {
-      const int ax = ix + (c & 1), ay = iy + ((c >> 1) & 1),
-                az = iz + (c >> 2);
-      acc += (double)g[(az * NN + ay) * NN + ax] * wx[c & 1] *
-             wy[(c >> 1) & 1] * wz[c >> 2];
+      for (int c = 0; c < 8; c++) {
+        const int ax = ix + (c & 1), ay = iy + ((c >> 1) & 1),
+                  az = iz + (c >> 2);
+        acc += (double)g[(az * NN + ay) * NN + ax] * wx[c & 1] *
+               wy[(c >> 1) & 1] * wz[c >> 2];
+      }
+      rf = (float)acc;
+      idx = __float_as_int(r.w);
+      if (rho_sorted) rho_sorted[k] = rf;
+      if (rho) rho[idx] = rf;
     }
-    const float rf = (float)acc;
-    const int32_t idx = __float_as_int(r.w);
-    if (rho_sorted) rho_sorted[k] = rf;
-    if (rho) rho[idx] = rf;
     if (seed_key && lab_sorted) {
-      const int32_t h = lab_sorted[k];
-      if (h >= 0 && h < nhalo) {
-        const unsigned long long key =
-            ((unsigned long long)__float_as_uint(rf) << 32) |
-            (unsigned long long)(0xFFFFFFFFu - (uint32_t)idx);
-        atomicMax(&seed_key[h], key);
+      const int32_t h = ok ? lab_sorted[k] : -1;
+      const bool mem = h >= 0 && h < nhalo;
+      const unsigned long long key =
+          mem ? (((unsigned long long)__float_as_uint(rf) << 32) |
+                 (unsigned long long)(0xFFFFFFFFu - (uint32_t)idx))
+              : 0ull;
+      // one atomic per distinct label of the wave (members of a halo are
+      // contiguous in cell order)
+      uint64_t todo = __ballot(mem);
+      while (todo) {
+        const int leader = __ffsll((unsigned long long)todo) - 1;
+        const int32_t hl = __shfl(h, leader, 64);
+        const bool mine = mem && h == hl;
+        unsigned long long v = mine ? key : 0ull;
+        for (int o = 32; o > 0; o >>= 1) {
+          const unsigned long long w = __shfl_xor(v, o, 64);
+          v = w > v ? w : v;
+        }
+        if ((int)(threadIdx.x & 63) == leader) atomicMax(&seed_key[hl], v);
+        todo &= ~__ballot(mine);
       }
     }
   }

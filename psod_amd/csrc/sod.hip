@@ -593,16 +593,40 @@ k_task_emit(SodCfg c, const int32_t *__restrict__ cell_start,
   }
 }
 
-// one wave per task: up to 64 candidates of one cell against the 27 cells
-// around that cell (every neighbour within 1 cell lies there)
+__device__ inline float wave_minf(float v) {
+  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ inline float wave_maxf(float v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// squared distance between two boxes (0 when they overlap)
+__device__ inline float box_gap2(const float *a0, const float *a1,
+                                 const float *b0, const float *b1) {
+  float s = 0.0f;
+  for (int d = 0; d < 3; d++) {
+    const float g = fmaxf(0.0f, fmaxf(b0[d] - a1[d], a0[d] - b1[d]));
+    s += g * g;
+  }
+  return s;
+}
+
+// One wave per task: up to 64 candidates of one cell against the 27 cells
+// around that cell (every neighbour within 1 cell lies there).  Dense cells
+// are in Morton order of their 1/8-cell sub-cells, so candidate groups and
+// neighbour chunks are compact; a chunk whose box lies beyond the reach of
+// the candidates' box is skipped.
 __global__ void __launch_bounds__(256)
 k_centre_task(SodCfg c, const int32_t *__restrict__ cell_start,
               const float4 *__restrict__ rec, const HaloPlan *__restrict__ plan,
               const int32_t *__restrict__ large, const int4 *__restrict__ tasks,
               const int32_t *__restrict__ ntask_total,
               uint64_t *__restrict__ taskres) {
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int32_t nt = *ntask_total;
+  const float cut = c.rs2f * 1.0001f + 1e-6f;  // conservative chunk culling
   for (int32_t t = blockIdx.x * 4 + w; t < nt; t += gridDim.x * 4) {
     const int4 tk = tasks[t];
     const HaloPlan pl = plan[large[tk.x]];
@@ -610,20 +634,33 @@ k_centre_task(SodCfg c, const int32_t *__restrict__ cell_start,
     walk_off(tk.y, 1, &ci, &cj, &ck);
     int32_t lc[3];
     key_at(c.nx, pl.cs, ci, cj, ck, lc);
-    float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+    float px = 0.f, py = 0.f, pz = 0.f;
+    int32_t pidx = 0;
     const bool have = l < tk.w;
     if (have) {
       const float4 r = rec[tk.z + l];
-      p.x = rel1(r.x, lc[0], ci, 0.0f);
-      p.y = rel1(r.y, lc[1], cj, 0.0f);
-      p.z = rel1(r.z, lc[2], ck, 0.0f);
-      p.w = r.w;
+      px = rel1(r.x, lc[0], ci, 0.0f);
+      py = rel1(r.y, lc[1], cj, 0.0f);
+      pz = rel1(r.z, lc[2], ck, 0.0f);
+      pidx = __float_as_int(r.w);
     }
+    float cb0[3], cb1[3];
+    cb0[0] = wave_minf(have ? px : INFINITY);
+    cb0[1] = wave_minf(have ? py : INFINITY);
+    cb0[2] = wave_minf(have ? pz : INFINITY);
+    cb1[0] = wave_maxf(have ? px : -INFINITY);
+    cb1[1] = wave_maxf(have ? py : -INFINITY);
+    cb1[2] = wave_maxf(have ? pz : -INFINITY);
     uint64_t acc = 0;
     for (int q = 0; q < 27; q++) {
       int ei, ej, ek;
       walk_off(q, 1, &ei, &ej, &ek);
       const int di = ci + ei, dj = cj + ej, dk = ck + ek;
+      {
+        const float nb0[3] = {(float)di, (float)dj, (float)dk};
+        const float nb1[3] = {(float)di + 1.f, (float)dj + 1.f, (float)dk + 1.f};
+        if (box_gap2(cb0, cb1, nb0, nb1) > cut) continue;
+      }
       int32_t ln[3];
       const int32_t key = key_at(c.nx, pl.cs, di, dj, dk, ln);
       const int32_t a = cell_start[key], b = cell_start[key + 1];
@@ -636,6 +673,17 @@ k_centre_task(SodCfg c, const int32_t *__restrict__ cell_start,
           vy = rel1(r.y, ln[1], dj, 0.0f);
           vz = rel1(r.z, ln[2], dk, 0.0f);
         }
+        if (m >= 16) {
+          float nb0[3], nb1[3];
+          nb0[0] = wave_minf(l < m ? vx : INFINITY);
+          nb0[1] = wave_minf(l < m ? vy : INFINITY);
+          nb0[2] = wave_minf(l < m ? vz : INFINITY);
+          nb1[0] = wave_maxf(l < m ? vx : -INFINITY);
+          nb1[1] = wave_maxf(l < m ? vy : -INFINITY);
+          nb1[2] = wave_maxf(l < m ? vz : -INFINITY);
+          if (box_gap2(cb0, cb1, nb0, nb1) > cut) continue;
+        }
+        uint32_t part = 0;  // <= 64 terms of <= 2^24
         for (int32_t u = 0; u < m; u++) {
           const float qx = __int_as_float(
               __builtin_amdgcn_readlane(__float_as_int(vx), u));
@@ -643,14 +691,14 @@ k_centre_task(SodCfg c, const int32_t *__restrict__ cell_start,
               __builtin_amdgcn_readlane(__float_as_int(vy), u));
           const float qz = __int_as_float(
               __builtin_amdgcn_readlane(__float_as_int(vz), u));
-          const float dx = qx - p.x, dy = qy - p.y, dz = qz - p.z;
+          const float dx = qx - px, dy = qy - py, dz = qz - pz;
           const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-          if (d2 <= c.rs2f) acc += sterm_fix(d2, c.eps2f);
+          part += d2 <= c.rs2f ? sterm_fix(d2, c.eps2f) : 0u;
         }
+        acc += part;
       }
     }
-    uint64_t key = have ? (((uint64_t)(uint32_t)tk.y << 32) |
-                           (uint32_t)__float_as_int(p.w))
+    uint64_t key = have ? (((uint64_t)(uint32_t)tk.y << 32) | (uint32_t)pidx)
                         : ~0ull;
     if (!have) acc = 0;
     wave_argmax(acc, key);
@@ -696,6 +744,7 @@ struct GrowShared {
   int32_t item;
   int32_t fill[MAXRANGE];
   int64_t off[MAXRANGE];
+  int64_t bump;
 };
 
 __device__ inline int32_t block_min_i32(int32_t v, GrowShared &G) {
@@ -849,11 +898,12 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
              const float *__restrict__ vel, const int64_t *__restrict__ ids,
              HaloPlan *__restrict__ plan, const int32_t *__restrict__ large,
              int32_t *__restrict__ counters, int32_t *__restrict__ hist_all,
-             const int64_t *__restrict__ hist_off, uint64_t *__restrict__ pool,
-             int64_t pool_cap, unsigned long long *__restrict__ pool_ctr,
-             psod_halo_out out) {
+             const int64_t *__restrict__ hist_off, uint64_t *__restrict__ pool_all,
+             int64_t pool_cap, psod_halo_out out) {
   __shared__ GrowShared G;
   const int32_t nlarge = counters[1];
+  // this workgroup's slice of the collection pool, reused for every halo
+  uint64_t *pool = pool_all + (int64_t)blockIdx.x * pool_cap;
   for (;;) {
     if (threadIdx.x == 0) G.item = atomicAdd(&counters[3], 1);
     __syncthreads();
@@ -872,6 +922,7 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
     res.centre_id = ids ? ids[centre] : (int64_t)centre;
     int32_t W = pl.w0 < 2 ? 2 : pl.w0;
     bool failed = false;
+    if (threadIdx.x == 0) G.bump = 0;
     double xcs[3];
     centre_scaled(pos, centre, c.nx, xcs);
     for (;;) {
@@ -969,14 +1020,15 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
       for (int r = 0; r < nr; r++)
         need += pool_slots(ldcg(&C[re[r] + 1]) - ldcg(&C[rs[r]]));
       if (threadIdx.x == 0) {
-        const unsigned long long o = atomicAdd(pool_ctr, (unsigned long long)need);
-        int64_t oo = (int64_t)o;
+        G.bump = 0;  // a retry starts the slice afresh
+        int64_t oo = 0;
         for (int r = 0; r < nr; r++) {
           G.off[r] = oo;
           G.fill[r] = 0;
           oo += pool_slots(ldcg(&C[re[r] + 1]) - ldcg(&C[rs[r]]));
         }
-        if ((int64_t)o + need > pool_cap) G.off[0] = -1;
+        G.bump = oo;
+        if (need > pool_cap) G.off[0] = -1;
       }
       __syncthreads();
       if (G.off[0] < 0) { failed = true; break; }
@@ -1061,9 +1113,9 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
       if (need2) {
         const int64_t n2 = pool_slots(ldcg(&C[b3 + 1]) - ldcg(&C[s2]));
         if (threadIdx.x == 0) {
-          const unsigned long long o =
-              atomicAdd(pool_ctr, (unsigned long long)n2);
-          G.off[MAXRANGE - 1] = (int64_t)o + n2 > pool_cap ? -1 : (int64_t)o;
+          const int64_t o = G.bump;
+          G.off[MAXRANGE - 1] = o + n2 > pool_cap ? -1 : o;
+          G.bump = o + n2;
           G.fill[MAXRANGE - 1] = 0;
         }
       }
@@ -1168,8 +1220,12 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
       res.status = PSOD_H_CLOSED | PSOD_H_LARGE | (acc ? PSOD_H_ACCEPTED : 0);
       break;
     }
-    if (failed) res.status = PSOD_H_LARGE | PSOD_H_UNCLOSED;
-    if (threadIdx.x == 0) store_result(out, h, c.nbins, res);
+    if (failed) res.status = PSOD_H_LARGE;  // band beyond the slice
+    if (threadIdx.x == 0) {
+      if (failed) atomicAdd(&counters[4], 1);
+      if (res.status & PSOD_H_UNCLOSED) atomicAdd(&counters[5], 1);
+      store_result(out, h, c.nbins, res);
+    }
     __syncthreads();
   }
 }
@@ -1280,16 +1336,27 @@ extern "C" int psod_sod_fof(psod_ctx *ctx, const psod_sod_params *p,
   PSOD_HIP(hipMemcpyAsync(&htot, hoff + nlarge, 8, hipMemcpyDeviceToHost, st));
   PSOD_HIP(hipStreamSynchronize(st));
   int32_t *hist = (int32_t *)psod_scratch(ctx, S_BINS, (size_t)htot * 4 + 64);
-  const int64_t pool_cap = (int64_t)1 << 25;
+  const unsigned ngrow = (unsigned)min(nlarge, ctx->num_cus * 2);
+  const int64_t pool_cap = (int64_t)1 << 20;  // keys per workgroup slice
   uint64_t *pool = (uint64_t *)psod_scratch(ctx, S_COLLECT,
-                                            (size_t)pool_cap * 8 + 64);
+                                            (size_t)pool_cap * 8 * ngrow);
   if (!hist || !pool) return PSOD_E_NOMEM;
-  unsigned long long *pctr = (unsigned long long *)(pool + pool_cap);
-  PSOD_TRY(psod_fill_u64(ctx, (uint64_t *)pctr, 1, 0));
-  k_grow_large<<<(unsigned)min(nlarge, ctx->num_cus * 2), GL_T, 0, st>>>(
+  k_grow_large<<<ngrow, GL_T, 0, st>>>(
       c, cat->cell_start, (const float4 *)cat->rec, cat->pos, cat->vel,
-      cat->ids, plan, large, counters, hist, hoff, pool, pool_cap, pctr, *out);
+      cat->ids, plan, large, counters, hist, hoff, pool, pool_cap, *out);
   PSOD_LAUNCH_CHECK();
+  PSOD_HIP(hipMemcpyAsync(hc, counters, 16, hipMemcpyDeviceToHost, st));
+  int32_t hf[2];
+  PSOD_HIP(hipMemcpyAsync(hf, counters + 4, 8, hipMemcpyDeviceToHost, st));
   PSOD_HIP(hipStreamSynchronize(st));
+  if (hf[0] > 0)
+    return psod_set_error(PSOD_E_CAPACITY,
+                          "sod: %d halos need more than %lld collected "
+                          "particles in one radial band", hf[0],
+                          (long long)pool_cap);
+  if (hf[1] > 0)
+    return psod_set_error(PSOD_E_UNCLOSED,
+                          "sod: %d halos do not close within half the box",
+                          hf[1]);
   return PSOD_OK;
 }

@@ -123,9 +123,11 @@ __device__ inline void centre_scaled(const float *pos, int32_t c, int32_t nx,
     xc[a] = (double)(pos[3 * (int64_t)c + a] * (float)nx);
 }
 
-// fixed point of one centre-search term (value <= 1/eps = 64 < 2^6)
-constexpr float SFIX = 16777216.0f;  // 2^24
-constexpr double SFIX_INV = 1.0 / 16777216.0;
+// Fixed point of one centre-search term (value <= 1/eps = 64 = 2^6): units
+// of 2^-18, so 64 terms fit a 32-bit partial sum.  Sums of fixed-point terms
+// do not depend on the order of the neighbours.
+constexpr float SFIX = 262144.0f;  // 2^18
+constexpr double SFIX_INV = 1.0 / 262144.0;
 __device__ inline uint32_t sterm_fix(float d2, float eps2) {
   return (uint32_t)(rsqrtf(d2 + eps2) * SFIX + 0.5f);
 }

@@ -187,6 +187,22 @@ def test_sod_fof_multi_delta(ctx, orc, c64):
     check_sod(o, r, 4)
 
 
+def test_plain_fp64_matches_oracle(ctx, orc, c64):
+    """The plain method is FP64 throughout: only the summation order of S
+    differs from the oracle, so every result matches to the last bit
+    except exact ties."""
+    o, r = sod_compare(ctx, orc, c64, deltas=(200.0, 500.0), plain=True)
+    assert np.all(o["status"] & 1)
+    assert (o["centre"] == r["centre"]).mean() > 0.999
+    same = o["centre"] == r["centre"]
+    np.testing.assert_array_equal(o["count"][same, :2], r["count"][same, :2])
+    np.testing.assert_array_equal(o["radius"][same, :2], r["radius"][same, :2])
+    np.testing.assert_array_equal(o["profile"][same], r["profile"][same])
+    np.testing.assert_array_equal(o["count2"][same], r["count2"][same])
+    np.testing.assert_allclose(o["rint"][same, :2], r["rint"][same, :2],
+                               rtol=1e-13)
+
+
 def test_sod_fof_c1(ctx, orc):
     c1 = catalogue(ctx, 128, 2000, 20000, 1)
     o, r = sod_compare(ctx, orc, c1)
@@ -260,3 +276,48 @@ def test_host_dropins_reference_grid(ctx, orc):
                                   np.lexsort((np.arange(len(rh)), -rh)))
     with pytest.raises(P.PsodError):
         P.cic(opos, 2 * nx, mass=np.arange(len(opos), dtype=np.float32))
+
+
+@pytest.mark.parametrize("nx,nclump,nhi,ncut", [(32, 60, 3000, 4),
+                                                 (64, 400, 20000, 4)])
+def test_spherover_greedy_matches_oracle(ctx, orc, nx, nclump, nhi, ncut):
+    """Reference-semantics greedy SOD (seeds by density, exclusion) on the
+    device against the oracle, record by record."""
+    import psod_amd as P
+    pos, vel, lab, ids, tab = orc.generate(
+        orc.gen_params(nx ** 3, nclump, nlo=20, nhi=nhi, seed=21),
+        with_vel=False)
+    grid = orc.cic(pos, 2 * nx, xxmin=-0.5)
+    rho = orc.cicm1(pos, grid, xxmin=-0.5)
+    _, isort, nnew = orc.tri_cic(rho)
+    r = orc.sod_greedy(pos, ids, isort, nnew,
+                       orc.greedy_params(nx, ncut=ncut, ref_order=0))
+    g = P.spherover(pos, ids, isort, nnew, nx, ncut=ncut)
+    assert g["status"] == r["status"] == P.OK
+    hr, hg = r["halos"], g["halos"]
+    assert len(hg["ipart"]) == len(hr["ipart"]) > 10
+    for k in ("ipart", "seed", "centre", "iamas", "masse", "masse2"):
+        np.testing.assert_array_equal(hg[k], hr[k], err_msg=k)
+    for k in ("rayon", "amass", "xc", "yc", "zc", "overdens"):
+        np.testing.assert_array_equal(hg[k], hr[k], err_msg=k)
+    np.testing.assert_array_equal(hg["accepted"], hr["accepted"])
+    np.testing.assert_array_equal(g["mass_bins"], r["mass_bins"])
+    np.testing.assert_allclose(g["rad_bins"], r["rad_bins"], rtol=1e-15)
+    np.testing.assert_array_equal(g["alive"], r["alive"])
+    np.testing.assert_array_equal(g["structure"], r["structure"])
+
+
+def test_spherover_abort_paths(ctx, orc):
+    import psod_amd as P
+    z = np.load(os.path.join(GOLD, "shells_crossing.npz"))
+    pos, nx = z["pos"], int(z["nx"])
+    rho = orc.cicm1(pos, orc.cic(pos, 2 * nx, xxmin=-0.5), xxmin=-0.5)
+    _, isort, nnew = orc.tri_cic(rho)
+    ids = np.arange(len(pos), dtype=np.int64)
+    g = P.spherover(pos, ids, isort, nnew, nx, mmin=20.0, nsel=50)
+    assert g["status"] == P.E_CAPACITY
+    g = P.spherover(pos, ids, isort, nnew, nx, mmin=20.0, seuil2=800.0)
+    assert g["status"] == P.OK
+    k = int(np.where(g["halos"]["centre"] == z["centre"])[0][0])
+    assert g["halos"]["masse"][k] == z["count"]
+    assert g["halos"]["masse2"][k] == 9
