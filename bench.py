@@ -49,6 +49,10 @@ def parse():
     ap.add_argument("--plain-sample", type=int, default=20000,
                     help="halos timed on the plain FP64 path (0: all)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--copy-check", action="store_true",
+                    help="after the timed steps, copy the CIC grid once "
+                         "(device to device, known size) to scale HBM "
+                         "traffic counters")
     return ap.parse_args()
 
 
@@ -196,6 +200,13 @@ def main():
                     achieved=round(achieved / 1e9, 1), peak=HBM_PEAK / 1e9,
                     unit="GB/s", frac=round(achieved / HBM_PEAK, 4),
                     traffic=traffic, algorithmic_bytes=b_dep)
+
+    if args.copy_check:
+        dst = torch.empty_like(grid)
+        dst.copy_(grid)
+        torch.cuda.synchronize()
+        log(f"copy check: {grid.numel() * 4} bytes read and written once")
+        del dst
 
     plain = None
     if not args.no_plain:

@@ -93,27 +93,138 @@ __device__ inline int find_seg(const SegTable &T, int32_t j) {
   return lo;
 }
 
-template <int R>
+// Tile of TB^3 blocks (8*TB cells per axis), R nodes per cell per axis.
+// Halo ranges: the boundary cells of the lower-neighbour blocks of the tile,
+// enumerated per neighbour block type; see tile_seg().
+template <int TB>
+struct TileSegs {
+  static constexpr int NA = TB * TB * 64;  // x-1 face: single cells
+  static constexpr int NB = TB * TB * 8;   // y-1 face: rows of 8 cells
+  static constexpr int NC = TB * TB;       // z-1 face: 64-cell slabs
+  static constexpr int ND = TB * 8;        // x-1,y-1 edge: single cells
+  static constexpr int NE = TB * 8;        // x-1,z-1 edge: single cells
+  static constexpr int NF = TB;            // y-1,z-1 edge: rows of 8
+  static constexpr int N = NA + NB + NC + ND + NE + NF + 1;
+};
+
+// s-th halo range of a tile: neighbour block offset (in blocks, relative to
+// the tile's first block), local cell key inside it and run length
+template <int TB>
+__device__ inline void tile_seg(int s, int *bi, int *bj, int *bk, int *loc,
+                                int *len) {
+  using T = TileSegs<TB>;
+  if (s < T::NA) {  // lx=7 of block (-1, j, k); cell (ly, lz)
+    const int j = (s >> 6) % TB, k = (s >> 6) / TB, c = s & 63;
+    *bi = -1; *bj = j; *bk = k;
+    *loc = ((c >> 3) * 8 + (c & 7)) * 8 + 7; *len = 1; return;
+  }
+  s -= T::NA;
+  if (s < T::NB) {  // ly=7 of block (i, -1, k); row lz
+    const int i = (s >> 3) % TB, k = (s >> 3) / TB, lz = s & 7;
+    *bi = i; *bj = -1; *bk = k;
+    *loc = (lz * 8 + 7) * 8; *len = 8; return;
+  }
+  s -= T::NB;
+  if (s < T::NC) {  // lz=7 of block (i, j, -1)
+    *bi = s % TB; *bj = s / TB; *bk = -1;
+    *loc = 7 * 64; *len = 64; return;
+  }
+  s -= T::NC;
+  if (s < T::ND) {  // lx=7, ly=7 of block (-1, -1, k); cell lz
+    *bi = -1; *bj = -1; *bk = s >> 3;
+    *loc = ((s & 7) * 8 + 7) * 8 + 7; *len = 1; return;
+  }
+  s -= T::ND;
+  if (s < T::NE) {  // lx=7, lz=7 of block (-1, j, -1); cell ly
+    *bi = -1; *bj = s >> 3; *bk = -1;
+    *loc = (7 * 8 + (s & 7)) * 8 + 7; *len = 1; return;
+  }
+  s -= T::NE;
+  if (s < T::NF) {  // ly=7, lz=7 of block (i, -1, -1)
+    *bi = s; *bj = -1; *bk = -1;
+    *loc = (7 * 8 + 7) * 8; *len = 8; return;
+  }
+  *bi = -1; *bj = -1; *bk = -1;  // corner cell
+  *loc = 511; *len = 1;
+}
+
+// Deposit of the (8*TB*R)^3 nodes owned by one tile.  Own particles come
+// from TB^2 contiguous runs of TB blocks; halo particles from the ranges of
+// tile_seg(), found by binary search on their prefix counts.
+template <int R, int TB>
 __global__ void __launch_bounds__(256)
 k_cic_deposit(const int32_t *__restrict__ cell_start,
               const float4 *__restrict__ rec, int32_t nx, int32_t ng,
               int32_t shift, double norm, float *__restrict__ grid) {
-  constexpr int NN = 8 * R;  // owned nodes per axis
+  constexpr int NN = 8 * TB * R;  // owned nodes per axis
+  constexpr int NS = TileSegs<TB>::N;
   __shared__ unsigned long long acc[NN * NN * NN];
-  __shared__ SegTable T;
+  __shared__ int32_t hstart[NS];
+  __shared__ int32_t hpre[NS + 1];
+  __shared__ int8_t hoff[NS][3];
+  __shared__ int32_t ostart[TB * TB], opre[TB * TB + 1];
+  __shared__ int32_t wtot[4];
   const int32_t nb = nx >> 3;
-  const int32_t blk = blockIdx.x;
-  const int bx = blk % nb, by = (blk / nb) % nb, bz = blk / (nb * nb);
+  const int32_t nt = nb / TB;  // tiles per axis
+  const int32_t tile = blockIdx.x;
+  const int tx = tile % nt, ty = (tile / nt) % nt, tz = tile / (nt * nt);
+  const int bx0 = tx * TB, by0 = ty * TB, bz0 = tz * TB;
   for (int i = threadIdx.x; i < NN * NN * NN; i += 256) acc[i] = 0ull;
-  build_segments(T, cell_start, nb, bx, by, bz);
-  const int32_t total = T.pre[NSEG];
-  for (int32_t j = threadIdx.x; j < total; j += 256) {
-    const int s = find_seg(T, j);
-    const float4 r = rec[T.start[s] + (j - T.pre[s])];
-    // position in CIC cells relative to this block's first node
-    const float qx = (float)R * (r.x + 8.0f * (float)T.off[s][0]);
-    const float qy = (float)R * (r.y + 8.0f * (float)T.off[s][1]);
-    const float qz = (float)R * (r.z + 8.0f * (float)T.off[s][2]);
+  // own runs: blocks (bx0..bx0+TB-1, by0+j, bz0+k) are consecutive keys
+  if (threadIdx.x < TB * TB) {
+    const int j = threadIdx.x % TB, k = threadIdx.x / TB;
+    const int32_t b0 = ((bz0 + k) * nb + (by0 + j)) * nb + bx0;
+    ostart[threadIdx.x] = cell_start[b0 * BS3];
+    opre[threadIdx.x + 1] = cell_start[(b0 + TB) * BS3];
+  }
+  // halo ranges and their prefix counts (blocks of 256 ranges)
+  int32_t carry = 0;
+  for (int base = 0; base < NS; base += 256) {
+    const int s = base + threadIdx.x;
+    int32_t cnt = 0;
+    if (s < NS) {
+      int bi, bj, bk, loc, len;
+      tile_seg<TB>(s, &bi, &bj, &bk, &loc, &len);
+      const int32_t nbk = (wrapi(bz0 + bk, nb) * nb + wrapi(by0 + bj, nb)) * nb +
+                          wrapi(bx0 + bi, nb);
+      const int32_t k0 = nbk * BS3 + loc;
+      const int32_t a = cell_start[k0];
+      cnt = cell_start[k0 + len] - a;
+      hstart[s] = a;
+      // offset of the neighbour block relative to the tile origin, blocks
+      hoff[s][0] = (int8_t)bi;
+      hoff[s][1] = (int8_t)bj;
+      hoff[s][2] = (int8_t)bk;
+    }
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int32_t inc = wave_incl_scan(cnt);
+    if (l == 63) wtot[w] = inc;
+    __syncthreads();
+    int32_t off = carry;
+    for (int i = 0; i < w; i++) off += wtot[i];
+    if (s < NS) hpre[s] = off + inc - cnt;
+    carry += wtot[0] + wtot[1] + wtot[2] + wtot[3];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    hpre[NS] = carry;
+    opre[0] = 0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    // prefix of the own runs (their ends were stored in opre[1..])
+    int32_t p = 0;
+    for (int r = 0; r < TB * TB; r++) {
+      const int32_t n = opre[r + 1] - ostart[r];
+      opre[r + 1] = p + n;
+      p += n;
+    }
+  }
+  __syncthreads();
+  auto deposit = [&](const float4 &r, float ox, float oy, float oz) {
+    // position in CIC cells relative to the tile's first node
+    const float qx = (float)R * (r.x + ox), qy = (float)R * (r.y + oy),
+                qz = (float)R * (r.z + oz);
     const float fx0 = floorf(qx), fy0 = floorf(qy), fz0 = floorf(qz);
     const int ix = (int)fx0, iy = (int)fy0, iz = (int)fz0;
     const float fx = qx - fx0, fy = qy - fy0, fz = qz - fz0;
@@ -121,18 +232,46 @@ k_cic_deposit(const int32_t *__restrict__ cell_start,
                 wz[2] = {1.0f - fz, fz};
 #pragma unroll
     for (int c = 0; c < 8; c++) {
-      const int ax = ix + (c & 1), ay = iy + ((c >> 1) & 1),
-                az = iz + (c >> 2);
+      const int ax = ix + (c & 1), ay = iy + ((c >> 1) & 1), az = iz + (c >> 2);
       if ((unsigned)ax < (unsigned)NN && (unsigned)ay < (unsigned)NN &&
           (unsigned)az < (unsigned)NN) {
         const float w = wx[c & 1] * wy[(c >> 1) & 1] * wz[c >> 2];
-        const unsigned long long q = __float2ull_rn(w * FIX_SCALE);
-        atomicAdd(&acc[(az * NN + ay) * NN + ax], q);
+        atomicAdd(&acc[(az * NN + ay) * NN + ax], __float2ull_rn(w * FIX_SCALE));
       }
     }
+  };
+  // own particles
+  const int32_t nown = opre[TB * TB];
+  for (int32_t t = threadIdx.x; t < nown; t += 256) {
+    int r = 0;
+    while (r + 1 < TB * TB && opre[r + 1] <= t) r++;
+    const int32_t k = ostart[r] + (t - opre[r]);
+    const float4 rv = rec[k];
+    // block of this record inside the tile: y/z from the run index, x from
+    // the block boundaries inside the run
+    const int j = r % TB, kk = r / TB;
+    int i = 0;
+    if (TB > 1) {
+      const int32_t b0 = ((bz0 + kk) * nb + (by0 + j)) * nb + bx0;
+      for (int q = 1; q < TB; q++)
+        if (k >= cell_start[(b0 + q) * BS3]) i = q;
+    }
+    deposit(rv, 8.0f * (float)i, 8.0f * (float)j, 8.0f * (float)kk);
+  }
+  // halo particles
+  const int32_t nh = hpre[NS];
+  for (int32_t t = threadIdx.x; t < nh; t += 256) {
+    int lo = 0, hi = NS;  // last s with hpre[s] <= t
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (hpre[mid] <= t) lo = mid; else hi = mid;
+    }
+    const float4 rv = rec[hstart[lo] + (t - hpre[lo])];
+    deposit(rv, 8.0f * (float)hoff[lo][0], 8.0f * (float)hoff[lo][1],
+            8.0f * (float)hoff[lo][2]);
   }
   __syncthreads();
-  const int32_t gx0 = bx * NN, gy0 = by * NN, gz0 = bz * NN;
+  const int32_t gx0 = bx0 * 8 * R, gy0 = by0 * 8 * R, gz0 = bz0 * 8 * R;
   for (int i = threadIdx.x; i < NN * NN * NN; i += 256) {
     const int ax = i % NN, ay = (i / NN) % NN, az = i / (NN * NN);
     const int32_t gx = (gx0 + ax + shift) & (ng - 1);
@@ -243,14 +382,18 @@ extern "C" int psod_cic_deposit(psod_ctx *ctx, int64_t npart, int32_t nx,
   if (npart <= 0 || !d_cell_start || !d_rec || !d_grid)
     return psod_set_error(PSOD_E_ARG, "psod_cic_deposit: bad arguments");
   const int32_t nb = nx / 8;
-  const unsigned nblk = (unsigned)nb * nb * nb;
   const double norm = (double)ng * (double)ng * (double)ng / (double)npart;
-  if (ng == nx)
-    k_cic_deposit<1><<<nblk, 256, 0, ctx->stream>>>(
+  if (ng == nx && nb % 2 == 0) {
+    const unsigned ntile = (unsigned)(nb / 2) * (nb / 2) * (nb / 2);
+    k_cic_deposit<1, 2><<<ntile, 256, 0, ctx->stream>>>(
         d_cell_start, (const float4 *)d_rec, nx, ng, shift, norm, d_grid);
-  else
-    k_cic_deposit<2><<<nblk, 256, 0, ctx->stream>>>(
+  } else if (ng == nx) {
+    k_cic_deposit<1, 1><<<(unsigned)nb * nb * nb, 256, 0, ctx->stream>>>(
         d_cell_start, (const float4 *)d_rec, nx, ng, shift, norm, d_grid);
+  } else {
+    k_cic_deposit<2, 1><<<(unsigned)nb * nb * nb, 256, 0, ctx->stream>>>(
+        d_cell_start, (const float4 *)d_rec, nx, ng, shift, norm, d_grid);
+  }
   PSOD_LAUNCH_CHECK();
   PSOD_HIP(hipStreamSynchronize(ctx->stream));
   return PSOD_OK;

@@ -1233,6 +1233,19 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
 // ---------------------------------------------------------------------------
 // host orchestration
 // ---------------------------------------------------------------------------
+__global__ void k_large_keys(const HaloPlan *__restrict__ plan,
+                             const int32_t *__restrict__ large, int32_t n,
+                             uint64_t *__restrict__ keys,
+                             uint32_t *__restrict__ vals) {
+  const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int32_t h = large[e];
+  const HaloPlan pl = plan[h];
+  const int64_t cost = (int64_t)max(pl.nmem, 0) + pl.n5;
+  keys[e] = (uint64_t)(0x7FFFFFFFll - min(cost, (int64_t)0x7FFFFFFF));
+  vals[e] = (uint32_t)h;
+}
+
 __global__ void k_hist_sizes(SodCfg c, const HaloPlan *__restrict__ plan,
                              const int32_t *__restrict__ large, int32_t nlarge,
                              int32_t *__restrict__ nb) {
@@ -1288,6 +1301,19 @@ extern "C" int psod_sod_fof(psod_ctx *ctx, const psod_sod_params *p,
   PSOD_HIP(hipStreamSynchronize(st));
   const int32_t nlarge = hc[1];
   if (nlarge == 0) return PSOD_OK;
+  // largest halos first, so that the persistent growth workgroups do not
+  // finish on a giant halo picked up last
+  {
+    uint64_t *keys = (uint64_t *)psod_scratch(ctx, S_KEYS, (size_t)nlarge * 8);
+    uint32_t *vals = (uint32_t *)psod_scratch(ctx, S_VALS, (size_t)nlarge * 4);
+    if (!keys || !vals) return PSOD_E_NOMEM;
+    k_large_keys<<<grid_for(nlarge, 256), 256, 0, st>>>(plan, large, nlarge,
+                                                        keys, vals);
+    PSOD_LAUNCH_CHECK();
+    PSOD_TRY(psod_radix_sort_u64(ctx, nlarge, keys, vals, 32));
+    PSOD_HIP(hipMemcpyAsync(large, vals, (size_t)nlarge * 4,
+                            hipMemcpyDeviceToDevice, st));
+  }
   // centre-search tasks of the large halos
   int32_t *ntask = (int32_t *)psod_scratch(ctx, S_MISC,
                                            (size_t)(3 * nlarge + 64) * 4);
