@@ -204,8 +204,10 @@ def main():
     if args.copy_check:
         dst = torch.empty_like(grid)
         dst.copy_(grid)
+        torch.mul(grid, 2.0, out=dst)  # elementwise kernel, same bytes
         torch.cuda.synchronize()
-        log(f"copy check: {grid.numel() * 4} bytes read and written once")
+        log(f"copy check: {grid.numel() * 4} bytes read and written once "
+            "(copy_, then mul)")
         del dst
 
     plain = None

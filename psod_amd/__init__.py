@@ -226,6 +226,14 @@ class Context:
                      _ptr(nmem), C.byref(ho)))
         return o
 
+    def sod_stats(self):
+        """Execution-path counts of the last sod_fof call: halos grown by
+        one wave, by one workgroup, re-run with larger collection slices,
+        and the keys per slice of that re-run."""
+        a = (C.c_int64 * 4)()
+        _lib.check(_L.psod_sod_stats(self.h, a))
+        return dict(wave=a[0], workgroup=a[1], retried=a[2], retry_slice=a[3])
+
     # ---- host drop-ins ---------------------------------------------------
     def host_cic(self, pos, ng, mass=None, xxmin=0.0):
         pos = np.ascontiguousarray(pos, np.float32)

@@ -15,7 +15,7 @@ F_VELOCITY = 1
 
 EXPORTS = [
     "psod_create", "psod_destroy", "psod_last_error", "psod_synchronize",
-    "psod_stream", "psod_scratch_bytes", "psod_cell_sort", "psod_cic_deposit",
+    "psod_stream", "psod_scratch_bytes", "psod_sod_stats", "psod_cell_sort", "psod_cic_deposit",
     "psod_cic_interp", "psod_seed_index", "psod_density_order",
     "psod_sod_fof", "psod_sod_fof_plain", "psod_label_counts",
     "psod_gen_clumps", "psod_generate", "psod_host_cic", "psod_host_cicm1",
@@ -100,6 +100,7 @@ def load():
         "psod_synchronize": (C.c_int, [v]),
         "psod_stream": (v, [v]),
         "psod_scratch_bytes": (i64, [v]),
+        "psod_sod_stats": (C.c_int, [v, P(i64)]),
         "psod_cell_sort": (C.c_int, [v, i64, v, v, i32, v, v, v, v]),
         "psod_cic_deposit": (C.c_int, [v, i64, i32, v, v, i32, i32, v]),
         "psod_cic_interp": (C.c_int, [v, i64, i32, v, v, i32, i32, v, v, v,

@@ -73,6 +73,12 @@ extern "C" int64_t psod_scratch_bytes(psod_ctx *c) {
   return s;
 }
 
+extern "C" int psod_sod_stats(psod_ctx *c, int64_t out[4]) {
+  if (!c || !out) return psod_set_error(PSOD_E_ARG, "psod_sod_stats: null argument");
+  for (int i = 0; i < 4; i++) out[i] = c->sod_stat[i];
+  return PSOD_OK;
+}
+
 int psod_use_device(psod_ctx *c) {
   if (!c) return psod_set_error(PSOD_E_ARG, "null context");
   PSOD_HIP(hipSetDevice(c->device));

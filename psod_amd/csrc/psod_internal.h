@@ -67,6 +67,7 @@ struct psod_ctx {
   void *slot[S_NSLOT];
   size_t cap[S_NSLOT];
   int num_cus;
+  int64_t sod_stat[4];  // see psod_sod_stats
 };
 
 // Returns a device buffer of at least `bytes` in `slot` (contents not kept

@@ -745,6 +745,7 @@ struct GrowShared {
   int32_t fill[MAXRANGE];
   int64_t off[MAXRANGE];
   int64_t bump;
+  int64_t need;  // slots the current halo's bands need
 };
 
 __device__ inline int32_t block_min_i32(int32_t v, GrowShared &G) {
@@ -899,17 +900,24 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
              HaloPlan *__restrict__ plan, const int32_t *__restrict__ large,
              int32_t *__restrict__ counters, int32_t *__restrict__ hist_all,
              const int64_t *__restrict__ hist_off, uint64_t *__restrict__ pool_all,
-             int64_t pool_cap, psod_halo_out out) {
+             int64_t pool_cap, const int32_t *__restrict__ elist,
+             int32_t *__restrict__ retry,
+             unsigned long long *__restrict__ needmax, psod_halo_out out) {
+  // Work items are positions e of the large list (or elist[item] when a
+  // list of positions is given).  A halo whose bands do not fit the slice
+  // is appended to `retry` (when given) with the slice size it needs
+  // folded into *needmax, for a second launch with larger slices.
   __shared__ GrowShared G;
-  const int32_t nlarge = counters[1];
+  const int32_t nitem = counters[1];
   // this workgroup's slice of the collection pool, reused for every halo
   uint64_t *pool = pool_all + (int64_t)blockIdx.x * pool_cap;
   for (;;) {
     if (threadIdx.x == 0) G.item = atomicAdd(&counters[3], 1);
     __syncthreads();
-    const int32_t e = G.item;
+    const int32_t item = G.item;
     __syncthreads();
-    if (e >= nlarge) break;
+    if (item >= nitem) break;
+    const int32_t e = elist ? elist[item] : item;
     const int64_t h = large[e];
     const HaloPlan pl = plan[h];
     const int32_t centre = pl.centre;
@@ -1028,6 +1036,7 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
           oo += pool_slots(ldcg(&C[re[r] + 1]) - ldcg(&C[rs[r]]));
         }
         G.bump = oo;
+        G.need = need;
         if (need > pool_cap) G.off[0] = -1;
       }
       __syncthreads();
@@ -1116,6 +1125,7 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
           const int64_t o = G.bump;
           G.off[MAXRANGE - 1] = o + n2 > pool_cap ? -1 : o;
           G.bump = o + n2;
+          G.need = o + n2;
           G.fill[MAXRANGE - 1] = 0;
         }
       }
@@ -1222,7 +1232,11 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
     }
     if (failed) res.status = PSOD_H_LARGE;  // band beyond the slice
     if (threadIdx.x == 0) {
-      if (failed) atomicAdd(&counters[4], 1);
+      if (failed) {
+        const int32_t k = atomicAdd(&counters[4], 1);
+        if (retry) retry[k] = e;
+        if (needmax) atomicMax(needmax, (unsigned long long)G.need);
+      }
       if (res.status & PSOD_H_UNCLOSED) atomicAdd(&counters[5], 1);
       store_result(out, h, c.nbins, res);
     }
@@ -1233,6 +1247,14 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
 // ---------------------------------------------------------------------------
 // host orchestration
 // ---------------------------------------------------------------------------
+// item counters of a re-run launch (layout of counters[0..7]) and its
+// largest needed slice
+__global__ void k_retry_reset(int32_t *__restrict__ c, int32_t nitem,
+                              unsigned long long *__restrict__ need) {
+  if (blockIdx.x == 0 && threadIdx.x < 8) c[threadIdx.x] = threadIdx.x == 1 ? nitem : 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *need = 0ull;
+}
+
 __global__ void k_large_keys(const HaloPlan *__restrict__ plan,
                              const int32_t *__restrict__ large, int32_t n,
                              uint64_t *__restrict__ keys,
@@ -1286,7 +1308,7 @@ extern "C" int psod_sod_fof(psod_ctx *ctx, const psod_sod_params *p,
                                            (size_t)(2 * nhalo + 64) * 4);
   if (!plan || !lists) return PSOD_E_NOMEM;
   int32_t *small = lists, *large = lists + nhalo, *counters = lists + 2 * nhalo;
-  PSOD_TRY(psod_fill_i32(ctx, counters, 16, 0));
+  PSOD_TRY(psod_fill_i32(ctx, counters, 40, 0));
   k_plan<<<grid_for(nhalo, 256), 256, 0, st>>>(
       c, nhalo, d_seed, d_nmem, cat->pos, cat->cell_start, plan, small, large,
       counters, *out);
@@ -1300,6 +1322,9 @@ extern "C" int psod_sod_fof(psod_ctx *ctx, const psod_sod_params *p,
   PSOD_HIP(hipMemcpyAsync(hc, counters, 16, hipMemcpyDeviceToHost, st));
   PSOD_HIP(hipStreamSynchronize(st));
   const int32_t nlarge = hc[1];
+  ctx->sod_stat[0] = nhalo - nlarge;
+  ctx->sod_stat[1] = nlarge;
+  ctx->sod_stat[2] = ctx->sod_stat[3] = 0;
   if (nlarge == 0) return PSOD_OK;
   // largest halos first, so that the persistent growth workgroups do not
   // finish on a giant halo picked up last
@@ -1316,10 +1341,11 @@ extern "C" int psod_sod_fof(psod_ctx *ctx, const psod_sod_params *p,
   }
   // centre-search tasks of the large halos
   int32_t *ntask = (int32_t *)psod_scratch(ctx, S_MISC,
-                                           (size_t)(3 * nlarge + 64) * 4);
+                                           (size_t)(5 * nlarge + 128) * 4);
   if (!ntask) return PSOD_E_NOMEM;
   int32_t *task0 = ntask + nlarge + 16;
   int32_t *ntot = task0 + nlarge + 16;
+  int32_t *retry = ntot + 16;  // positions of halos that need larger slices
   k_task_count<<<grid_for(nlarge, 256), 256, 0, st>>>(c, cat->cell_start,
                                                        plan, large, nlarge,
                                                        ntask);
@@ -1363,23 +1389,72 @@ extern "C" int psod_sod_fof(psod_ctx *ctx, const psod_sod_params *p,
   PSOD_HIP(hipStreamSynchronize(st));
   int32_t *hist = (int32_t *)psod_scratch(ctx, S_BINS, (size_t)htot * 4 + 64);
   const unsigned ngrow = (unsigned)min(nlarge, ctx->num_cus * 2);
-  const int64_t pool_cap = (int64_t)1 << 20;  // keys per workgroup slice
+  // keys per workgroup slice; PSOD_SLICE_KEYS lowers it (tests of the
+  // retry launch on small catalogues)
+  int64_t pool_cap = (int64_t)1 << 20;
+  if (const char *e = getenv("PSOD_SLICE_KEYS")) {
+    const long long v = atoll(e);
+    if (v >= 64 && v < pool_cap) pool_cap = v;
+  }
   uint64_t *pool = (uint64_t *)psod_scratch(ctx, S_COLLECT,
                                             (size_t)pool_cap * 8 * ngrow);
   if (!hist || !pool) return PSOD_E_NOMEM;
+  // counters[16..17]: largest slice a failed halo needs; counters[24..31]
+  // and [32..33]: the same for a re-run launch
+  unsigned long long *needmax = (unsigned long long *)(counters + 16);
   k_grow_large<<<ngrow, GL_T, 0, st>>>(
       c, cat->cell_start, (const float4 *)cat->rec, cat->pos, cat->vel,
-      cat->ids, plan, large, counters, hist, hoff, pool, pool_cap, *out);
+      cat->ids, plan, large, counters, hist, hoff, pool, pool_cap, nullptr,
+      retry, needmax, *out);
   PSOD_LAUNCH_CHECK();
-  PSOD_HIP(hipMemcpyAsync(hc, counters, 16, hipMemcpyDeviceToHost, st));
   int32_t hf[2];
+  unsigned long long hneed = 0;
   PSOD_HIP(hipMemcpyAsync(hf, counters + 4, 8, hipMemcpyDeviceToHost, st));
+  PSOD_HIP(hipMemcpyAsync(&hneed, needmax, 8, hipMemcpyDeviceToHost, st));
   PSOD_HIP(hipStreamSynchronize(st));
-  if (hf[0] > 0)
-    return psod_set_error(PSOD_E_CAPACITY,
-                          "sod: %d halos need more than %lld collected "
-                          "particles in one radial band", hf[0],
-                          (long long)pool_cap);
+  // Further launches for the halos whose bands exceeded the slice: fewer
+  // workgroups with larger slices.  A halo re-run from its first reach may
+  // need more than its first failure showed (a larger reach later), so the
+  // slice grows at least 4x per launch.
+  int32_t *cur = retry, *nxt = retry + nlarge + 16;
+  int32_t nfail = hf[0];
+  int64_t cap2 = pool_cap;
+  for (int pass = 0; nfail > 0; pass++) {
+    const int64_t budget = (int64_t)32 << 30;  // bytes for all slices
+    cap2 = std::max<int64_t>((int64_t)hneed, 4 * cap2);
+    if (pass >= 6 || cap2 * 8 > budget)
+      return psod_set_error(PSOD_E_CAPACITY,
+                            "sod: %d halos need more than %lld collected "
+                            "particles in their radial bands", nfail,
+                            (long long)std::min<int64_t>(cap2, budget / 8));
+    if (getenv("PSOD_VERBOSE"))
+      fprintf(stderr, "psod: %d of %d large halos re-run with %lld-key "
+              "slices\n", nfail, nlarge, (long long)cap2);
+    const int64_t nfit = budget / (cap2 * 8);
+    const unsigned ngrow2 = (unsigned)std::min<int64_t>(
+        std::min<int64_t>(nfail, nfit), (int64_t)ctx->num_cus * 2);
+    uint64_t *pool2 = (uint64_t *)psod_scratch(ctx, S_COLLECT,
+                                               (size_t)cap2 * 8 * ngrow2);
+    if (!pool2) return PSOD_E_NOMEM;
+    if (pass == 0) ctx->sod_stat[2] = nfail;
+    ctx->sod_stat[3] = cap2;
+    int32_t *c2 = counters + 24;
+    unsigned long long *need2 = (unsigned long long *)(counters + 32);
+    k_retry_reset<<<1, 64, 0, st>>>(c2, nfail, need2);
+    PSOD_LAUNCH_CHECK();
+    k_grow_large<<<ngrow2, GL_T, 0, st>>>(
+        c, cat->cell_start, (const float4 *)cat->rec, cat->pos, cat->vel,
+        cat->ids, plan, large, c2, hist, hoff, pool2, cap2, cur, nxt, need2,
+        *out);
+    PSOD_LAUNCH_CHECK();
+    int32_t hf2[2];
+    PSOD_HIP(hipMemcpyAsync(hf2, c2 + 4, 8, hipMemcpyDeviceToHost, st));
+    PSOD_HIP(hipMemcpyAsync(&hneed, need2, 8, hipMemcpyDeviceToHost, st));
+    PSOD_HIP(hipStreamSynchronize(st));
+    nfail = hf2[0];
+    hf[1] += hf2[1];
+    std::swap(cur, nxt);
+  }
   if (hf[1] > 0)
     return psod_set_error(PSOD_E_UNCLOSED,
                           "sod: %d halos do not close within half the box",

@@ -203,6 +203,34 @@ def test_plain_fp64_matches_oracle(ctx, orc, c64):
                                rtol=1e-13)
 
 
+def test_sod_retry_with_larger_slices(ctx, c64, monkeypatch):
+    """Halos whose radial bands overflow a workgroup's collection slice are
+    re-run with larger slices; the results do not depend on the slice."""
+    import psod_amd as P
+    nx, N = c64["nx"], c64["N"]
+    cat = ctx.cell_sort(c64["pos"], nx, labels=c64["lab"])
+    grid = ctx.cic_deposit(cat, nx)
+    seeds = ctx.seeds(cat, grid, c64["nh"])
+    nmem = ctx.label_counts(c64["lab"], c64["nh"])
+    p = P.sod_params(nx, N, deltas=(200.0, 500.0), delta2=800.0,
+                     velocity=True)
+
+    def run():
+        o = ctx.sod_fof(p, cat, c64["pos"], seeds, nmem, vel=c64["vel"],
+                        ids=c64["ids"])
+        return {k: host(v) for k, v in o.items()}, ctx.sod_stats()
+
+    a, sa = run()
+    monkeypatch.setenv("PSOD_SLICE_KEYS", "64")
+    b, sb = run()
+    monkeypatch.delenv("PSOD_SLICE_KEYS")
+    print("stats", sa, sb, flush=True)
+    assert sa["retried"] == 0 and sb["retried"] > 0
+    assert sb["workgroup"] == sa["workgroup"] > 0
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+
+
 def test_sod_fof_c1(ctx, orc):
     c1 = catalogue(ctx, 128, 2000, 20000, 1)
     o, r = sod_compare(ctx, orc, c1)
