@@ -26,6 +26,7 @@
 #include "sod_common.h"
 
 #include <cstring>
+#include <vector>
 
 int psod_fill_i32(psod_ctx *c, int32_t *p, int64_t n, int32_t v);
 int psod_fill_u64(psod_ctx *c, uint64_t *p, int64_t n, uint64_t v);
@@ -730,7 +731,14 @@ k_centre_reduce(HaloPlan *__restrict__ plan, const int32_t *__restrict__ large,
 // ---------------------------------------------------------------------------
 constexpr int GL_T = 256;
 constexpr int SORT_CAP = 4096;
-constexpr int MAXRANGE = PSOD_MAXDELTA + 1;
+// segments of possible crossing bins per threshold; ranges of collected
+// bins (the last slot holds the second threshold's band)
+constexpr int SEGCAP = 4;
+constexpr int MAXRANGE = PSOD_MAXDELTA * SEGCAP + 1;
+// radial histogram of one workgroup: HIST_MIN..HIST_MAX bins (powers of
+// two, a multiple of GL_T) and the total
+constexpr int32_t HIST_MIN = 4096, HIST_MAX = 1 << 20;
+constexpr int64_t HIST_SLICE = HIST_MAX + 64;
 
 struct GrowShared {
   uint64_t skey[SORT_CAP];
@@ -744,6 +752,11 @@ struct GrowShared {
   int32_t item;
   int32_t fill[MAXRANGE];
   int64_t off[MAXRANGE];
+  int32_t rs[MAXRANGE], re[MAXRANGE];  // bin ranges, ascending, disjoint
+  int64_t k0[MAXRANGE], k1[MAXRANGE];  // ranks before / at the range end
+  int32_t nr;
+  int32_t sg_s[PSOD_MAXDELTA][SEGCAP], sg_e[PSOD_MAXDELTA][SEGCAP],
+      sg_x[PSOD_MAXDELTA][SEGCAP], nseg[PSOD_MAXDELTA];
   int64_t bump;
   int64_t need;  // slots the current halo's bands need
 };
@@ -780,6 +793,38 @@ __device__ inline double block_sum_f64(double v, GrowShared &G) {
   for (int i = 0; i < GL_T / 64; i++) r += G.red_d[i];
   __syncthreads();
   return r;
+}
+
+// first b in [lo, hi] with pred(b) (INT32_MAX if none), scanning upward in
+// chunks of 4 * GL_T bins so that an early hit ends the scan
+template <typename P>
+__device__ inline int32_t block_first(int32_t lo, int32_t hi, P &&pred,
+                                      GrowShared &G) {
+  for (int32_t base = lo; base <= hi; base += 4 * GL_T) {
+    int32_t m = INT32_MAX;
+    for (int u = 0; u < 4; u++) {
+      const int32_t b = base + u * GL_T + (int32_t)threadIdx.x;
+      if (m == INT32_MAX && b <= hi && pred(b)) m = b;
+    }
+    m = block_min_i32(m, G);
+    if (m != INT32_MAX) return m;
+  }
+  return INT32_MAX;
+}
+// last b in [lo, hi] with pred(b) (-1 if none), scanning downward
+template <typename P>
+__device__ inline int32_t block_last(int32_t lo, int32_t hi, P &&pred,
+                                     GrowShared &G) {
+  for (int32_t top = hi; top >= lo; top -= 4 * GL_T) {
+    int32_t m = -1;
+    for (int u = 0; u < 4; u++) {
+      const int32_t b = top - u * GL_T - (int32_t)threadIdx.x;
+      if (m < 0 && b >= lo && pred(b)) m = b;
+    }
+    m = block_max_i32(m, G);
+    if (m >= 0) return m;
+  }
+  return -1;
 }
 
 // Calls f(d2, idx, rec, di, dj, dk, lc) for every particle of the cells at
@@ -841,12 +886,34 @@ __device__ inline int32_t rbin(float d2, float scale, int32_t nb) {
   return b < nb ? b : nb - 1;
 }
 
-// bins whose edges may be compared with exact radii: widen by 1e-5
-__device__ inline double edge_lo(int32_t b, int32_t nb, double W) {
-  return W * cbrt((double)b / (double)nb) * (1.0 - 1e-5) - 1e-6;
+// Edges of bin b in r^3, widened by 4e-5 (relative) so that every particle
+// binned by rbin() lies inside them whatever the float rounding of d2^1.5
+__device__ inline double edge3_lo(int32_t b, int32_t nb, double W3) {
+  return W3 * ((double)b / (double)nb) * (1.0 - 4e-5);
 }
-__device__ inline double edge_hi(int32_t b, int32_t nb, double W) {
-  return W * cbrt((double)(b + 1) / (double)nb) * (1.0 + 1e-5) + 1e-6;
+__device__ inline double edge3_hi(int32_t b, int32_t nb, double W3) {
+  return W3 * ((double)(b + 1) / (double)nb) * (1.0 + 4e-5) + 1e-12;
+}
+// r^3 at which k particles have overdensity delta (od_k(k, m, r) = delta)
+__device__ inline double rad3_for(int64_t k, double mpart, double delta) {
+  return ((double)k * mpart) / (delta * (2.0 / 3.0 * TWOPI_REF));
+}
+
+// particles in the cells at offsets [-W, W]^3 (cell counts only)
+__device__ inline int64_t cube_count(const SodCfg &c, GrowShared &G,
+                                     const int32_t *cell_start,
+                                     const CellRef &cr, int W) {
+  const int side = 2 * W + 1;
+  const int32_t ncell = side * side * side;
+  int64_t n = 0;
+  for (int32_t q = threadIdx.x; q < ncell; q += GL_T) {
+    int di, dj, dk;
+    walk_off(q, W, &di, &dj, &dk);
+    int32_t lc[3];
+    const int32_t key = key_at(c.nx, cr.c, di, dj, dk, lc);
+    n += cell_start[key + 1] - cell_start[key];
+  }
+  return block_sum_i64(n, G);
 }
 
 // Pool slots reserved for n collected keys: n itself when the keys are
@@ -899,7 +966,7 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
              const float *__restrict__ vel, const int64_t *__restrict__ ids,
              HaloPlan *__restrict__ plan, const int32_t *__restrict__ large,
              int32_t *__restrict__ counters, int32_t *__restrict__ hist_all,
-             const int64_t *__restrict__ hist_off, uint64_t *__restrict__ pool_all,
+             uint64_t *__restrict__ pool_all,
              int64_t pool_cap, const int32_t *__restrict__ elist,
              int32_t *__restrict__ retry,
              unsigned long long *__restrict__ needmax, psod_halo_out out) {
@@ -909,8 +976,10 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
   // folded into *needmax, for a second launch with larger slices.
   __shared__ GrowShared G;
   const int32_t nitem = counters[1];
-  // this workgroup's slice of the collection pool, reused for every halo
+  // this workgroup's slices of the collection pool and of the radial
+  // histogram, reused for every halo
   uint64_t *pool = pool_all + (int64_t)blockIdx.x * pool_cap;
+  int32_t *hist = hist_all + (int64_t)blockIdx.x * HIST_SLICE;
   for (;;) {
     if (threadIdx.x == 0) G.item = atomicAdd(&counters[3], 1);
     __syncthreads();
@@ -922,8 +991,6 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
     const HaloPlan pl = plan[h];
     const int32_t centre = pl.centre;
     const CellRef cr = ref_of_particle(pos, centre, c.nx);
-    int32_t *hist = hist_all + hist_off[e];
-    const int32_t nb = (int32_t)(hist_off[e + 1] - hist_off[e]) - 1;
     HaloResult res;
     zero_result(res);
     res.centre = centre;
@@ -940,6 +1007,10 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
         break;
       }
       const float W2 = (float)W * (float)W;
+      // bins uniform in r^3, about 4 particles of the cube per bin
+      const int64_t ncube = cube_count(c, G, cell_start, cr, W);
+      int32_t nb = HIST_MIN;
+      while ((int64_t)nb * 4 < ncube && nb < HIST_MAX) nb <<= 1;
       const float scale = (float)nb / ((float)W * (float)W * (float)W);
       for (int32_t b = threadIdx.x; b <= nb; b += GL_T) hist[b] = 0;
       __threadfence_block();
@@ -955,100 +1026,141 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
       ng = block_sum_i64(ng, G);
       __threadfence();
       __syncthreads();
-      // exclusive scan in place: hist[b] = #(bin < b), hist[nb] = total
+      // exclusive scan in place: hist[b] = #(bin < b), hist[nb] = total;
+      // thread t owns the nb / GL_T consecutive bins from t * nb / GL_T
       {
-        int32_t carry = 0;
-        for (int32_t base = 0; base <= nb; base += GL_T) {
-          const int32_t b = base + threadIdx.x;
-          const int32_t v = b < nb ? ldcg(&hist[b]) : 0;
-          const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-          int32_t tot;
-          const int32_t ex = wave_excl(v, &tot);
-          if (l == 0) G.red_i[w] = tot;
-          __syncthreads();
-          int32_t off = 0, all = 0;
-          for (int i = 0; i < GL_T / 64; i++) {
-            if (i < w) off += G.red_i[i];
-            all += G.red_i[i];
-          }
-          if (b <= nb) hist[b] = carry + off + ex;
-          carry += all;
-          __syncthreads();
+        const int32_t per = nb / GL_T;
+        int32_t *hp = hist + threadIdx.x * per;
+        int32_t sum = 0;
+        for (int32_t i = 0; i < per; i++) sum += ldcg(&hp[i]);
+        const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+        int32_t tot;
+        const int32_t ex = wave_excl(sum, &tot);
+        if (l == 0) G.red_i[w] = tot;
+        __syncthreads();
+        int32_t run = ex;
+        for (int i = 0; i < w; i++) run += G.red_i[i];
+        for (int32_t i = 0; i < per; i++) {
+          const int32_t v = ldcg(&hp[i]);
+          hp[i] = run;
+          run += v;
         }
+        if (threadIdx.x == GL_T - 1) hist[nb] = run;
+        __threadfence_block();
+        __syncthreads();
       }
-      __threadfence_block();
-      __syncthreads();
       const int32_t *C = hist;
-      const double Wd = (double)W;
-      // ---- crossing bands of every threshold -----------------------------
-      int32_t b1[PSOD_MAXDELTA], b2[PSOD_MAXDELTA], bp[PSOD_MAXDELTA];
+      const double Wd = (double)W, W3 = Wd * Wd * Wd;
+      // ---- bins that may hold each threshold's first crossing --------------
+      // A bin is possible for delta when the smallest admissible rank k in it
+      // could have od_k <= delta at the bin's outer edge, and definite when
+      // it must at the inner edge; the crossing lies in a possible bin no
+      // later than the first definite one.  The possible bins are collected
+      // as up to SEGCAP segments, each extended down to the previous
+      // non-empty bin, which holds rank k-1 when k opens the segment.
       bool ok = true;
+      auto bin_k = [&](int32_t b, int64_t *kmin) {
+        const int32_t cb = ldcg(&C[b]), ce = ldcg(&C[b + 1]);
+        *kmin = max((int64_t)cb + 1, (int64_t)c.nmin_count);
+        return *kmin <= ce;
+      };
+      auto nonempty = [&](int32_t b) { return ldcg(&C[b + 1]) > ldcg(&C[b]); };
       for (int di = 0; di < c.ndelta && ok; di++) {
-        int32_t first_any = INT32_MAX, first_def = INT32_MAX;
-        for (int32_t b = threadIdx.x; b < nb; b += GL_T) {
-          const int32_t cb = ldcg(&C[b]), ce = ldcg(&C[b + 1]);
-          const int64_t kmin = max((int64_t)cb + 1, (int64_t)c.nmin_count);
-          if (kmin > ce) continue;
-          const double f = rad_for(kmin, c.mpart, c.delta[di]);
-          if (f <= edge_lo(b, nb, Wd)) {
-            first_def = min(first_def, b);
-            first_any = min(first_any, b);
-          } else if (f < edge_hi(b, nb, Wd)) {
-            first_any = min(first_any, b);
+        const double dlt = c.delta[di];
+        auto definite = [&](int32_t b) {
+          int64_t k;
+          return bin_k(b, &k) && rad3_for(k, c.mpart, dlt) <= edge3_lo(b, nb, W3);
+        };
+        auto possible = [&](int32_t b) {
+          int64_t k;
+          return bin_k(b, &k) && rad3_for(k, c.mpart, dlt) < edge3_hi(b, nb, W3);
+        };
+        const int32_t b2 = block_first(0, nb - 1, definite, G);
+        if (b2 == INT32_MAX) { ok = false; break; }
+        int32_t last_e = -1, x = 0, ns = 0;
+        for (;;) {
+          const int32_t s0 = block_first(x, b2, possible, G);
+          int32_t e = b2;
+          if (ns < SEGCAP - 1) {
+            const int32_t q = block_first(
+                s0 + 1, b2, [&](int32_t b) { return nonempty(b) && !possible(b); },
+                G);
+            if (q != INT32_MAX) e = q - 1;
           }
+          const int32_t xs =
+              s0 > last_e + 1 ? block_last(last_e + 1, s0 - 1, nonempty, G) : -1;
+          if (threadIdx.x == 0) {
+            G.sg_s[di][ns] = s0;
+            G.sg_e[di][ns] = e;
+            G.sg_x[di][ns] = xs >= 0 ? xs : s0;
+          }
+          ns++;
+          if (e >= b2) break;
+          last_e = e;
+          x = e + 1;
         }
-        b1[di] = block_min_i32(first_any, G);
-        b2[di] = block_min_i32(first_def, G);
-        if (b2[di] == INT32_MAX) { ok = false; break; }
-        int32_t prev = -1;
-        for (int32_t b = threadIdx.x; b < b1[di]; b += GL_T)
-          if (ldcg(&C[b + 1]) > ldcg(&C[b])) prev = max(prev, b);
-        bp[di] = block_max_i32(prev, G);
-        if (bp[di] < 0) bp[di] = b1[di];
+        if (threadIdx.x == 0) G.nseg[di] = ns;
+        __syncthreads();
       }
       if (!ok) {
         W = 2 * W;
         __syncthreads();
         continue;
       }
-      // ---- merge the bands into disjoint ranges --------------------------
-      int32_t rs[MAXRANGE], re[MAXRANGE], nr = 0;
-      for (int di = 0; di < c.ndelta; di++) {
-        int32_t s = bp[di], t = b2[di];
-        bool merged = false;
-        for (int r = 0; r < nr; r++)
-          if (!(t < rs[r] - 1 || s > re[r] + 1)) {
-            rs[r] = min(rs[r], s);
-            re[r] = max(re[r], t);
-            merged = true;
-          }
-        if (!merged) { rs[nr] = s; re[nr] = t; nr++; }
-      }
-      int64_t need = 0;
-      for (int r = 0; r < nr; r++)
-        need += pool_slots(ldcg(&C[re[r] + 1]) - ldcg(&C[rs[r]]));
+      // ---- disjoint ranges of bins to collect -------------------------------
       if (threadIdx.x == 0) {
-        G.bump = 0;  // a retry starts the slice afresh
+        int32_t nr = 0;
+        for (int di = 0; di < c.ndelta; di++)
+          for (int j = 0; j < G.nseg[di]; j++) {
+            int32_t a = G.sg_x[di][j], t = G.sg_e[di][j];
+            // absorb every range that overlaps or touches [a, t]
+            int32_t w = 0;
+            for (int r = 0; r < nr; r++) {
+              if (!(t < G.rs[r] - 1 || a > G.re[r] + 1)) {
+                a = min(a, G.rs[r]);
+                t = max(t, G.re[r]);
+              } else {
+                G.rs[w] = G.rs[r];
+                G.re[w] = G.re[r];
+                w++;
+              }
+            }
+            G.rs[w] = a;
+            G.re[w] = t;
+            nr = w + 1;
+          }
+        // ascending order of the ranges (ranks then ascend with them)
+        for (int r = 1; r < nr; r++)
+          for (int q = r; q > 0 && G.rs[q - 1] > G.rs[q]; q--) {
+            const int32_t ts = G.rs[q], te = G.re[q];
+            G.rs[q] = G.rs[q - 1];
+            G.re[q] = G.re[q - 1];
+            G.rs[q - 1] = ts;
+            G.re[q - 1] = te;
+          }
+        G.nr = nr;
         int64_t oo = 0;
         for (int r = 0; r < nr; r++) {
+          G.k0[r] = ldcg(&C[G.rs[r]]);
+          G.k1[r] = ldcg(&C[G.re[r] + 1]);
           G.off[r] = oo;
           G.fill[r] = 0;
-          oo += pool_slots(ldcg(&C[re[r] + 1]) - ldcg(&C[rs[r]]));
+          oo += pool_slots(G.k1[r] - G.k0[r]);
         }
-        G.bump = oo;
-        G.need = need;
-        if (need > pool_cap) G.off[0] = -1;
+        G.bump = oo;  // a re-run starts the slice afresh
+        G.need = oo;
       }
       __syncthreads();
-      if (G.off[0] < 0) { failed = true; break; }
-      // ---- pass 2: collect the bands --------------------------------------
+      const int nr = G.nr;
+      if (G.need > pool_cap) { failed = true; break; }
+      // ---- pass 2: collect the ranges ---------------------------------------
       cube_walk(c, G, cell_start, rec, cr, W,
                 [&](float d2, int32_t idx, const float4 &, int, int, int,
                     const int32_t *) {
                   if (!(d2 < W2)) return;
                   const int32_t b = rbin(d2, scale, nb);
                   for (int r = 0; r < nr; r++)
-                    if (b >= rs[r] && b <= re[r]) {
+                    if (b >= G.rs[r] && b <= G.re[r]) {
                       const int32_t s = atomicAdd(&G.fill[r], 1);
                       pool[G.off[r] + s] = d2key(d2, idx);
                     }
@@ -1056,33 +1168,39 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
       __threadfence_block();
       __syncthreads();
       for (int r = 0; r < nr; r++)
-        block_sort_pool(pool + G.off[r], ldcg(&C[re[r] + 1]) - ldcg(&C[rs[r]]),
-                        G);
+        block_sort_pool(pool + G.off[r], (int32_t)(G.k1[r] - G.k0[r]), G);
+      // key of rank k (1-based) among the collected ranges
+      auto key_rank = [&](int64_t k) {
+        int r = 0;
+        while (r + 1 < nr && k > G.k1[r]) r++;
+        return pool[G.off[r] + (k - G.k0[r] - 1)];
+      };
       // ---- exact first crossing ------------------------------------------
       int64_t kst[PSOD_MAXDELTA];
       uint64_t keyR = 0;
       for (int di = 0; di < c.ndelta; di++) {
-        int r = 0;
-        for (int q = 0; q < nr; q++)
-          if (bp[di] >= rs[q] && b2[di] <= re[q]) r = q;
-        const int64_t k0 = ldcg(&C[rs[r]]);  // rank before the range
-        const int64_t klo = max((int64_t)ldcg(&C[b1[di]]) + 1,
-                                (int64_t)c.nmin_count);
-        const int64_t khi = ldcg(&C[b2[di] + 1]);
         int32_t firstk = INT32_MAX;
-        for (int64_t k = klo + threadIdx.x; k <= khi; k += GL_T) {
-          const double rk = (double)sqrtf(key_d2(pool[G.off[r] + (k - k0 - 1)]));
-          if (od_k(k, c.mpart, rk) <= c.delta[di]) firstk = min(firstk, (int32_t)k);
+        for (int j = 0; j < G.nseg[di] && firstk == INT32_MAX; j++) {
+          const int64_t klo = max((int64_t)ldcg(&C[G.sg_s[di][j]]) + 1,
+                                  (int64_t)c.nmin_count);
+          const int64_t khi = ldcg(&C[G.sg_e[di][j] + 1]);
+          int32_t fk = INT32_MAX;
+          for (int64_t k = klo + threadIdx.x; k <= khi; k += GL_T) {
+            const double rk = (double)sqrtf(key_d2(key_rank(k)));
+            if (od_k(k, c.mpart, rk) <= c.delta[di]) {
+              fk = (int32_t)k;
+              break;
+            }
+          }
+          firstk = block_min_i32(fk, G);
         }
-        firstk = block_min_i32(firstk, G);
         kst[di] = firstk == INT32_MAX ? -1 : firstk;
         if (kst[di] < 0) { ok = false; break; }
-        const double r1 = ref_dist(
-            pos, key_idx(pool[G.off[r] + (kst[di] - 1 - k0 - 1)]), xcs, c.nx);
-        const double r2 = ref_dist(
-            pos, key_idx(pool[G.off[r] + (kst[di] - k0 - 1)]), xcs, c.nx);
+        const uint64_t k1 = key_rank(kst[di] - 1);
+        const double r1 = ref_dist(pos, key_idx(k1), xcs, c.nx);
+        const double r2 = ref_dist(pos, key_idx(key_rank(kst[di])), xcs, c.nx);
         finish_delta(c, di, kst[di], r1, r2, res);
-        if (di == 0) keyR = pool[G.off[r] + (kst[di] - 1 - k0 - 1)];
+        if (di == 0) keyR = k1;
       }
       if (!ok) {
         W = 2 * W;
@@ -1106,11 +1224,11 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
           const int64_t klo = max((int64_t)cb + 1, (int64_t)2);
           const int64_t kmax = min((int64_t)ce, k0s);
           if (kmax < klo) continue;
-          const double g = rad_for(kmax, c.mpart, c.delta2);
-          if (g >= edge_hi(b, nb, Wd)) {
+          const double g = rad3_for(kmax, c.mpart, c.delta2);
+          if (g >= edge3_hi(b, nb, W3)) {
             last_def = max(last_def, b);
             last_any = max(last_any, b);
-          } else if (g > edge_lo(b, nb, Wd)) {
+          } else if (g > edge3_lo(b, nb, W3)) {
             last_any = max(last_any, b);
           }
         }
@@ -1268,28 +1386,6 @@ __global__ void k_large_keys(const HaloPlan *__restrict__ plan,
   vals[e] = (uint32_t)h;
 }
 
-__global__ void k_hist_sizes(SodCfg c, const HaloPlan *__restrict__ plan,
-                             const int32_t *__restrict__ large, int32_t nlarge,
-                             int32_t *__restrict__ nb) {
-  const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nlarge) return;
-  const HaloPlan pl = plan[large[e]];
-  const double w = (double)(pl.w0 < 2 ? 2 : pl.w0);
-  const double est = 2.0 * ((double)pl.nmem + 4.19 * w * w * w * c.nbar);
-  int32_t n = 4096;
-  while (n < est && n < (1 << 22)) n <<= 1;
-  nb[e] = n + 1;
-}
-
-__global__ void k_i32_to_i64_scan(const int32_t *__restrict__ ex,
-                                  const int32_t *__restrict__ v, int32_t n,
-                                  int64_t *__restrict__ out) {
-  // ex is an int32 exclusive scan whose total fits in int32
-  const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e < n) out[e] = ex[e];
-  if (e == n - 1) out[n] = (int64_t)ex[e] + v[e];
-}
-
 extern "C" int psod_sod_fof(psod_ctx *ctx, const psod_sod_params *p,
                             const psod_catalogue *cat, int64_t nhalo,
                             const int32_t *d_seed, const int32_t *d_nmem,
@@ -1370,31 +1466,15 @@ extern "C" int psod_sod_fof(psod_ctx *ctx, const psod_sod_params *p,
                                                            nlarge, tres);
     PSOD_LAUNCH_CHECK();
   }
-  // histogram slices
-  int32_t *nbv = (int32_t *)psod_scratch(ctx, S_PLAIN2,
-                                         (size_t)(2 * nlarge + 64) * 4 +
-                                             (size_t)(nlarge + 8) * 8);
-  if (!nbv) return PSOD_E_NOMEM;
-  int32_t *nbx = nbv + nlarge + 16;
-  int64_t *hoff = (int64_t *)(nbx + nlarge + 16);
-  k_hist_sizes<<<grid_for(nlarge, 256), 256, 0, st>>>(c, plan, large, nlarge,
-                                                      nbv);
-  PSOD_LAUNCH_CHECK();
-  PSOD_TRY(psod_scan_i32(ctx, nbv, nbx, nlarge, nullptr));
-  k_i32_to_i64_scan<<<grid_for(nlarge, 256), 256, 0, st>>>(nbx, nbv, nlarge,
-                                                           hoff);
-  PSOD_LAUNCH_CHECK();
-  int64_t htot = 0;
-  PSOD_HIP(hipMemcpyAsync(&htot, hoff + nlarge, 8, hipMemcpyDeviceToHost, st));
-  PSOD_HIP(hipStreamSynchronize(st));
-  int32_t *hist = (int32_t *)psod_scratch(ctx, S_BINS, (size_t)htot * 4 + 64);
   const unsigned ngrow = (unsigned)min(nlarge, ctx->num_cus * 2);
+  int32_t *hist = (int32_t *)psod_scratch(ctx, S_BINS,
+                                          (size_t)ngrow * HIST_SLICE * 4);
   // keys per workgroup slice; PSOD_SLICE_KEYS lowers it (tests of the
   // retry launch on small catalogues)
   int64_t pool_cap = (int64_t)1 << 20;
   if (const char *e = getenv("PSOD_SLICE_KEYS")) {
     const long long v = atoll(e);
-    if (v >= 64 && v < pool_cap) pool_cap = v;
+    if (v >= 16 && v < pool_cap) pool_cap = v;
   }
   uint64_t *pool = (uint64_t *)psod_scratch(ctx, S_COLLECT,
                                             (size_t)pool_cap * 8 * ngrow);
@@ -1404,7 +1484,7 @@ extern "C" int psod_sod_fof(psod_ctx *ctx, const psod_sod_params *p,
   unsigned long long *needmax = (unsigned long long *)(counters + 16);
   k_grow_large<<<ngrow, GL_T, 0, st>>>(
       c, cat->cell_start, (const float4 *)cat->rec, cat->pos, cat->vel,
-      cat->ids, plan, large, counters, hist, hoff, pool, pool_cap, nullptr,
+      cat->ids, plan, large, counters, hist, pool, pool_cap, nullptr,
       retry, needmax, *out);
   PSOD_LAUNCH_CHECK();
   int32_t hf[2];
@@ -1427,9 +1507,17 @@ extern "C" int psod_sod_fof(psod_ctx *ctx, const psod_sod_params *p,
                             "sod: %d halos need more than %lld collected "
                             "particles in their radial bands", nfail,
                             (long long)std::min<int64_t>(cap2, budget / 8));
-    if (getenv("PSOD_VERBOSE"))
+    if (getenv("PSOD_VERBOSE")) {
       fprintf(stderr, "psod: %d of %d large halos re-run with %lld-key "
-              "slices\n", nfail, nlarge, (long long)cap2);
+              "slices; halos:", nfail, nlarge, (long long)cap2);
+      const int32_t nshow = std::min<int32_t>(nfail, 64);
+      std::vector<int32_t> pe(nshow), ph(nlarge);
+      PSOD_HIP(hipMemcpy(pe.data(), cur, nshow * 4, hipMemcpyDeviceToHost));
+      PSOD_HIP(hipMemcpy(ph.data(), large, (size_t)nlarge * 4,
+                         hipMemcpyDeviceToHost));
+      for (int32_t i = 0; i < nshow; i++) fprintf(stderr, " %d", ph[pe[i]]);
+      fprintf(stderr, "\n");
+    }
     const int64_t nfit = budget / (cap2 * 8);
     const unsigned ngrow2 = (unsigned)std::min<int64_t>(
         std::min<int64_t>(nfail, nfit), (int64_t)ctx->num_cus * 2);
@@ -1444,7 +1532,7 @@ extern "C" int psod_sod_fof(psod_ctx *ctx, const psod_sod_params *p,
     PSOD_LAUNCH_CHECK();
     k_grow_large<<<ngrow2, GL_T, 0, st>>>(
         c, cat->cell_start, (const float4 *)cat->rec, cat->pos, cat->vel,
-        cat->ids, plan, large, c2, hist, hoff, pool2, cap2, cur, nxt, need2,
+        cat->ids, plan, large, c2, hist, pool2, cap2, cur, nxt, need2,
         *out);
     PSOD_LAUNCH_CHECK();
     int32_t hf2[2];

@@ -221,7 +221,7 @@ def test_sod_retry_with_larger_slices(ctx, c64, monkeypatch):
         return {k: host(v) for k, v in o.items()}, ctx.sod_stats()
 
     a, sa = run()
-    monkeypatch.setenv("PSOD_SLICE_KEYS", "64")
+    monkeypatch.setenv("PSOD_SLICE_KEYS", "16")
     b, sb = run()
     monkeypatch.delenv("PSOD_SLICE_KEYS")
     print("stats", sa, sb, flush=True)
