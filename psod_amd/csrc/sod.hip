@@ -614,22 +614,39 @@ __device__ inline float box_gap2(const float *a0, const float *a1,
   return s;
 }
 
+// Upper bound of the term of any pair at squared distance >= g2 (the float
+// d2 of a pair is at least g2 up to a few ulps, hence the 2e-5 shrink; +4
+// covers non-monotone rounding of the reciprocal square root)
+__device__ inline uint32_t sterm_bound(float g2, const SodCfg &c) {
+  const float g = g2 * (1.0f - 2e-5f);
+  return g <= c.rs2f ? sterm_fix(g, c.eps2f) + 4u : 0u;
+}
+
 // One wave per task: up to 64 candidates of one cell against the 27 cells
 // around that cell (every neighbour within 1 cell lies there).  Dense cells
 // are in Morton order of their 1/8-cell sub-cells, so candidate groups and
 // neighbour chunks are compact; a chunk whose box lies beyond the reach of
 // the candidates' box is skipped.
+//
+// Branch and bound, exact: phase 0 runs the tasks of the seed cell (which
+// holds the density peak) and folds their best S into hbest[halo]; phase 1
+// runs the other tasks, first bounding each candidate's S by the chunk
+// counts times the term at the candidate-to-chunk-box distance, and skips
+// the task when no bound reaches hbest (such a task can neither hold the
+// maximum nor tie with it).
 __global__ void __launch_bounds__(256)
 k_centre_task(SodCfg c, const int32_t *__restrict__ cell_start,
               const float4 *__restrict__ rec, const HaloPlan *__restrict__ plan,
               const int32_t *__restrict__ large, const int4 *__restrict__ tasks,
               const int32_t *__restrict__ ntask_total,
-              uint64_t *__restrict__ taskres) {
+              uint64_t *__restrict__ taskres, int phase,
+              unsigned long long *__restrict__ hbest) {
   const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int32_t nt = *ntask_total;
   const float cut = c.rs2f * 1.0001f + 1e-6f;  // conservative chunk culling
   for (int32_t t = blockIdx.x * 4 + w; t < nt; t += gridDim.x * 4) {
     const int4 tk = tasks[t];
+    if ((tk.y == 13) != (phase == 0)) continue;  // 13: the seed cell
     const HaloPlan pl = plan[large[tk.x]];
     int ci, cj, ck;
     walk_off(tk.y, 1, &ci, &cj, &ck);
@@ -652,6 +669,57 @@ k_centre_task(SodCfg c, const int32_t *__restrict__ cell_start,
     cb1[0] = wave_maxf(have ? px : -INFINITY);
     cb1[1] = wave_maxf(have ? py : -INFINITY);
     cb1[2] = wave_maxf(have ? pz : -INFINITY);
+    if (phase == 1) {
+      uint64_t ub = 0;
+      for (int q = 0; q < 27; q++) {
+        int ei, ej, ek;
+        walk_off(q, 1, &ei, &ej, &ek);
+        const int di = ci + ei, dj = cj + ej, dk = ck + ek;
+        {
+          const float nb0[3] = {(float)di, (float)dj, (float)dk};
+          const float nb1[3] = {(float)di + 1.f, (float)dj + 1.f,
+                                (float)dk + 1.f};
+          if (box_gap2(cb0, cb1, nb0, nb1) > cut) continue;
+        }
+        int32_t ln[3];
+        const int32_t key = key_at(c.nx, pl.cs, di, dj, dk, ln);
+        const int32_t a = cell_start[key], b = cell_start[key + 1];
+        for (int32_t s = a; s < b; s += 64) {
+          const int32_t m = min(64, b - s);
+          float vx = 0.f, vy = 0.f, vz = 0.f;
+          if (l < m) {
+            const float4 r = rec[s + l];
+            vx = rel1(r.x, ln[0], di, 0.0f);
+            vy = rel1(r.y, ln[1], dj, 0.0f);
+            vz = rel1(r.z, ln[2], dk, 0.0f);
+          }
+          const float n0x = wave_minf(l < m ? vx : INFINITY);
+          const float n0y = wave_minf(l < m ? vy : INFINITY);
+          const float n0z = wave_minf(l < m ? vz : INFINITY);
+          const float n1x = wave_maxf(l < m ? vx : -INFINITY);
+          const float n1y = wave_maxf(l < m ? vy : -INFINITY);
+          const float n1z = wave_maxf(l < m ? vz : -INFINITY);
+          const float gx = fmaxf(0.0f, fmaxf(n0x - px, px - n1x));
+          const float gy = fmaxf(0.0f, fmaxf(n0y - py, py - n1y));
+          const float gz = fmaxf(0.0f, fmaxf(n0z - pz, pz - n1z));
+          ub += (uint64_t)m * sterm_bound(gx * gx + gy * gy + gz * gz, c);
+        }
+      }
+      uint64_t ubmax = have ? ub : 0;
+      for (int o = 32; o > 0; o >>= 1) {
+        const uint64_t v = __shfl_xor(ubmax, o, 64);
+        ubmax = v > ubmax ? v : ubmax;
+      }
+      const unsigned long long best =
+          __hip_atomic_load(&hbest[tk.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (ubmax < best) {
+        if (l == 0) {
+          taskres[2 * (int64_t)t] = 0;
+          taskres[2 * (int64_t)t + 1] = ~0ull;
+        }
+        continue;
+      }
+    }
     uint64_t acc = 0;
     for (int q = 0; q < 27; q++) {
       int ei, ej, ek;
@@ -706,6 +774,7 @@ k_centre_task(SodCfg c, const int32_t *__restrict__ cell_start,
     if (l == 0) {
       taskres[2 * (int64_t)t] = acc;
       taskres[2 * (int64_t)t + 1] = key;
+      atomicMax(&hbest[tk.x], (unsigned long long)acc);
     }
   }
 }
@@ -734,7 +803,9 @@ constexpr int SORT_CAP = 4096;
 // segments of possible crossing bins per threshold; ranges of collected
 // bins (the last slot holds the second threshold's band)
 constexpr int SEGCAP = 4;
-constexpr int MAXRANGE = PSOD_MAXDELTA * SEGCAP + 1;
+constexpr int32_t GAP_KEYS = 512;  // runs this close share a segment
+constexpr int R2 = PSOD_MAXDELTA * SEGCAP;  // first slot of the delta2 segments
+constexpr int MAXRANGE = R2 + SEGCAP;
 // radial histogram of one workgroup: HIST_MIN..HIST_MAX bins (powers of
 // two, a multiple of GL_T) and the total
 constexpr int32_t HIST_MIN = 4096, HIST_MAX = 1 << 20;
@@ -796,13 +867,14 @@ __device__ inline double block_sum_f64(double v, GrowShared &G) {
 }
 
 // first b in [lo, hi] with pred(b) (INT32_MAX if none), scanning upward in
-// chunks of 4 * GL_T bins so that an early hit ends the scan
+// chunks of SCAN_U * GL_T bins so that an early hit ends the scan
+constexpr int SCAN_U = 16;
 template <typename P>
 __device__ inline int32_t block_first(int32_t lo, int32_t hi, P &&pred,
                                       GrowShared &G) {
-  for (int32_t base = lo; base <= hi; base += 4 * GL_T) {
+  for (int32_t base = lo; base <= hi; base += SCAN_U * GL_T) {
     int32_t m = INT32_MAX;
-    for (int u = 0; u < 4; u++) {
+    for (int u = 0; u < SCAN_U; u++) {
       const int32_t b = base + u * GL_T + (int32_t)threadIdx.x;
       if (m == INT32_MAX && b <= hi && pred(b)) m = b;
     }
@@ -815,9 +887,9 @@ __device__ inline int32_t block_first(int32_t lo, int32_t hi, P &&pred,
 template <typename P>
 __device__ inline int32_t block_last(int32_t lo, int32_t hi, P &&pred,
                                      GrowShared &G) {
-  for (int32_t top = hi; top >= lo; top -= 4 * GL_T) {
+  for (int32_t top = hi; top >= lo; top -= SCAN_U * GL_T) {
     int32_t m = -1;
-    for (int u = 0; u < 4; u++) {
+    for (int u = 0; u < SCAN_U; u++) {
       const int32_t b = top - u * GL_T - (int32_t)threadIdx.x;
       if (m < 0 && b >= lo && pred(b)) m = b;
     }
@@ -1087,14 +1159,23 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
                 G);
             if (q != INT32_MAX) e = q - 1;
           }
-          const int32_t xs =
-              s0 > last_e + 1 ? block_last(last_e + 1, s0 - 1, nonempty, G) : -1;
+          // a run close behind the previous segment (few keys between)
+          // extends it instead of opening a segment
+          const bool join =
+              ns > 0 && ldcg(&C[s0]) - ldcg(&C[last_e + 1]) <= GAP_KEYS;
+          const int32_t xs = (!join && s0 > last_e + 1)
+                                 ? block_last(last_e + 1, s0 - 1, nonempty, G)
+                                 : -1;
           if (threadIdx.x == 0) {
-            G.sg_s[di][ns] = s0;
-            G.sg_e[di][ns] = e;
-            G.sg_x[di][ns] = xs >= 0 ? xs : s0;
+            if (join) {
+              G.sg_e[di][ns - 1] = e;
+            } else {
+              G.sg_s[di][ns] = s0;
+              G.sg_e[di][ns] = e;
+              G.sg_x[di][ns] = xs >= 0 ? xs : s0;
+            }
           }
-          ns++;
+          if (!join) ns++;
           if (e >= b2) break;
           last_e = e;
           x = e + 1;
@@ -1214,37 +1295,72 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
         __syncthreads();
         continue;
       }
-      // ---- second-threshold band (last k above delta2) ---------------------
-      int32_t b3 = -1, b4 = -1;
+      // ---- second-threshold bins (last k above delta2) ----------------------
+      // A bin may hold a rank k <= k0s with od_k > delta2 when its largest
+      // such rank could at the bin's inner edge (possible), and must at the
+      // outer edge (definite).  The answer lies in a possible bin between
+      // the last definite bin (or bin 0) and the last possible bin; those
+      // bins are collected as up to SEGCAP segments found downward, the
+      // lowest one extended to the floor.
       const int64_t k0s = kst[0];
+      int ns2 = 0;
       if (c.delta2 > 0.0) {
-        int32_t last_any = -1, last_def = -1;
-        for (int32_t b = threadIdx.x; b < nb; b += GL_T) {
+        auto kmax2 = [&](int32_t b, int64_t *kmax) {
           const int32_t cb = ldcg(&C[b]), ce = ldcg(&C[b + 1]);
-          const int64_t klo = max((int64_t)cb + 1, (int64_t)2);
-          const int64_t kmax = min((int64_t)ce, k0s);
-          if (kmax < klo) continue;
-          const double g = rad3_for(kmax, c.mpart, c.delta2);
-          if (g >= edge3_hi(b, nb, W3)) {
-            last_def = max(last_def, b);
-            last_any = max(last_any, b);
-          } else if (g > edge3_lo(b, nb, W3)) {
-            last_any = max(last_any, b);
+          *kmax = min((int64_t)ce, k0s);
+          return *kmax >= max((int64_t)cb + 1, (int64_t)2);
+        };
+        auto possible2 = [&](int32_t b) {
+          int64_t k;
+          return kmax2(b, &k) && rad3_for(k, c.mpart, c.delta2) > edge3_lo(b, nb, W3);
+        };
+        auto definite2 = [&](int32_t b) {
+          int64_t k;
+          return kmax2(b, &k) && rad3_for(k, c.mpart, c.delta2) >= edge3_hi(b, nb, W3);
+        };
+        // ranks above k0s do not count: no bin beyond the one of rank k0s
+        const int32_t bk0 = rbin(key_d2(key_rank(k0s)), scale, nb);
+        const int32_t b3 = block_last(0, bk0, possible2, G);
+        if (b3 >= 0) {
+          const int32_t b4 = block_last(0, b3, definite2, G);
+          const int32_t floor2 = b4 >= 0 ? b4 : 0;
+          int32_t x = b3;
+          while (x >= floor2) {
+            const int32_t e = block_last(floor2, x, possible2, G);
+            if (e < 0) break;
+            int32_t s0 = floor2;
+            if (ns2 < SEGCAP - 1) {
+              const int32_t q = block_last(
+                  floor2, e - 1,
+                  [&](int32_t b) { return nonempty(b) && !possible2(b); }, G);
+              if (q >= 0) s0 = q + 1;
+            }
+            // join the segment above when few keys lie between
+            const bool join = ns2 > 0 && ldcg(&C[x + 1]) - ldcg(&C[e + 1]) <= GAP_KEYS;
+            if (threadIdx.x == 0) {
+              if (join) {
+                G.rs[R2 + ns2 - 1] = s0;
+              } else {
+                G.rs[R2 + ns2] = s0;
+                G.re[R2 + ns2] = e;
+              }
+            }
+            if (!join) ns2++;
+            if (s0 <= floor2) break;
+            x = s0 - 1;
           }
         }
-        b3 = block_max_i32(last_any, G);
-        b4 = block_max_i32(last_def, G);
-      }
-      const bool need2 = b3 >= 0;
-      const int32_t s2 = b4 >= 0 ? b4 : 0;
-      if (need2) {
-        const int64_t n2 = pool_slots(ldcg(&C[b3 + 1]) - ldcg(&C[s2]));
         if (threadIdx.x == 0) {
-          const int64_t o = G.bump;
-          G.off[MAXRANGE - 1] = o + n2 > pool_cap ? -1 : o;
-          G.bump = o + n2;
-          G.need = o + n2;
-          G.fill[MAXRANGE - 1] = 0;
+          int64_t o = G.bump;
+          for (int j = 0; j < ns2; j++) {
+            G.k0[R2 + j] = ldcg(&C[G.rs[R2 + j]]);
+            G.k1[R2 + j] = ldcg(&C[G.re[R2 + j] + 1]);
+            G.off[R2 + j] = o;
+            G.fill[R2 + j] = 0;
+            o += pool_slots(G.k1[R2 + j] - G.k0[R2 + j]);
+          }
+          G.bump = o;
+          G.need = o;
         }
       }
       // profile radii
@@ -1253,7 +1369,7 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
         G.pcnt[j] = 0;
       }
       __syncthreads();
-      if (need2 && G.off[MAXRANGE - 1] < 0) { failed = true; break; }
+      if (G.need > pool_cap) { failed = true; break; }
       // ---- pass 3: profile, centre of mass, boundary, band of delta2 -------
       const double rlast = G.prad[c.nbins - 1];
       double sp[3] = {0, 0, 0}, sv[3] = {0, 0, 0};
@@ -1281,12 +1397,13 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
                       for (int d = 0; d < 3; d++)
                         sv[d] += (double)vel[3 * (int64_t)idx + d];
                   }
-                  if (need2) {
+                  if (ns2 > 0) {
                     const int32_t b = rbin(d2, scale, nb);
-                    if (b >= s2 && b <= b3) {
-                      const int32_t s = atomicAdd(&G.fill[MAXRANGE - 1], 1);
-                      pool[G.off[MAXRANGE - 1] + s] = d2key(d2, idx);
-                    }
+                    for (int j = 0; j < ns2; j++)
+                      if (b >= G.rs[R2 + j] && b <= G.re[R2 + j]) {
+                        const int32_t s = atomicAdd(&G.fill[R2 + j], 1);
+                        pool[G.off[R2 + j] + s] = d2key(d2, idx);
+                      }
                   }
                 });
       nbd = block_sum_i64(nbd, G);
@@ -1313,30 +1430,34 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
       // ---- delta2 --------------------------------------------------------
       if (c.delta2 > 0.0) {
         int64_t best = 0;
-        if (need2) {
+        uint64_t bkey = 0;
+        if (ns2 > 0) {
           __threadfence_block();
           __syncthreads();
-          const int32_t n2 = ldcg(&C[b3 + 1]) - ldcg(&C[s2]);
-          uint64_t *p2 = pool + G.off[MAXRANGE - 1];
-          block_sort_pool(p2, n2, G);
-          const int64_t kb = ldcg(&C[s2]);
-          int32_t bestk = -1;
-          for (int32_t t = threadIdx.x; t < n2; t += GL_T) {
-            const int64_t k = kb + t + 1;
-            if (k < 2 || k > k0s) continue;
-            const double rk = (double)sqrtf(key_d2(p2[t]));
-            if (od_k(k, c.mpart, rk) > c.delta2) bestk = max(bestk, (int32_t)k);
+          // segments from the top down: the first one holding a rank above
+          // delta2 holds the last such rank
+          for (int j = 0; j < ns2 && best == 0; j++) {
+            const int32_t n2 = (int32_t)(G.k1[R2 + j] - G.k0[R2 + j]);
+            uint64_t *p2 = pool + G.off[R2 + j];
+            block_sort_pool(p2, n2, G);
+            const int64_t kb = G.k0[R2 + j];
+            int32_t bestk = -1;
+            for (int32_t t = threadIdx.x; t < n2; t += GL_T) {
+              const int64_t k = kb + t + 1;
+              if (k < 2 || k > k0s) continue;
+              const double rk = (double)sqrtf(key_d2(p2[t]));
+              if (od_k(k, c.mpart, rk) > c.delta2) bestk = max(bestk, (int32_t)k);
+            }
+            bestk = block_max_i32(bestk, G);
+            if (bestk > 0) {
+              best = bestk;
+              bkey = p2[bestk - kb - 1];
+            }
           }
-          bestk = block_max_i32(bestk, G);
-          if (bestk > 0) best = bestk;
         }
         res.count2 = (int32_t)best;
         if (best > 0) {
-          // rank `best` lies in the collected band of delta2
-          const int64_t kb = ldcg(&C[s2]);
-          const double rb = ref_dist(
-              pos, key_idx(pool[G.off[MAXRANGE - 1] + (best - kb - 1)]), xcs,
-              c.nx);
+          const double rb = ref_dist(pos, key_idx(bkey), xcs, c.nx);
           res.overdens2 = rb > 1e-30 ? od_k(best, c.mpart, rb) : 0.0;
         }
       }
@@ -1436,9 +1557,12 @@ extern "C" int psod_sod_fof(psod_ctx *ctx, const psod_sod_params *p,
                             hipMemcpyDeviceToDevice, st));
   }
   // centre-search tasks of the large halos
-  int32_t *ntask = (int32_t *)psod_scratch(ctx, S_MISC,
-                                           (size_t)(5 * nlarge + 128) * 4);
+  int32_t *ntask = (int32_t *)psod_scratch(
+      ctx, S_MISC, (size_t)(5 * nlarge + 128) * 4 + (size_t)nlarge * 8 + 256);
   if (!ntask) return PSOD_E_NOMEM;
+  // best S found so far per large halo (branch and bound of the search)
+  unsigned long long *hbest = (unsigned long long *)(
+      (((uintptr_t)(ntask + 5 * nlarge + 128)) + 255) & ~(uintptr_t)255);
   int32_t *task0 = ntask + nlarge + 16;
   int32_t *ntot = task0 + nlarge + 16;
   int32_t *retry = ntot + 16;  // positions of halos that need larger slices
@@ -1458,10 +1582,14 @@ extern "C" int psod_sod_fof(psod_ctx *ctx, const psod_sod_params *p,
     k_task_emit<<<grid_for(nlarge, 256), 256, 0, st>>>(
         c, cat->cell_start, plan, large, nlarge, ntask, task0, tasks);
     PSOD_LAUNCH_CHECK();
-    k_centre_task<<<grid_for((int64_t)nt, 4, (int64_t)ctx->num_cus * 16), 256,
-                    0, st>>>(c, cat->cell_start, (const float4 *)cat->rec,
-                             plan, large, tasks, ntot, tres);
-    PSOD_LAUNCH_CHECK();
+    PSOD_TRY(psod_fill_u64(ctx, (uint64_t *)hbest, nlarge, 0));
+    for (int phase = 0; phase < 2; phase++) {
+      k_centre_task<<<grid_for((int64_t)nt, 4, (int64_t)ctx->num_cus * 16),
+                      256, 0, st>>>(c, cat->cell_start,
+                                    (const float4 *)cat->rec, plan, large,
+                                    tasks, ntot, tres, phase, hbest);
+      PSOD_LAUNCH_CHECK();
+    }
     k_centre_reduce<<<grid_for(nlarge, 256), 256, 0, st>>>(plan, large,
                                                            nlarge, tres);
     PSOD_LAUNCH_CHECK();
