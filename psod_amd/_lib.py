@@ -5,7 +5,9 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIBPATH = os.path.join(HERE, "libpsod.so")
+# PSOD_LIBRARY selects another build of the same library (A/B timing of
+# builds); by default the in-tree build is loaded
+LIBPATH = os.environ.get("PSOD_LIBRARY") or os.path.join(HERE, "libpsod.so")
 
 OK, E_ARG, E_DEVICE, E_CAPACITY, E_UNCLOSED, E_NOMEM = 0, 1, 2, 3, 4, 5
 MAXDELTA = 4

@@ -1,97 +1,31 @@
 // cic.hip -- a2 CIC deposit, a3 CIC^-1 (+ FoF seeds), a4 density order.
 //
-// Deposit (main file:1077-1265).  One workgroup owns the (8R)^3 nodes of
-// one 8^3-cell block (R = ng/nx in {1, 2}).  A node receives mass from the
-// cells on both sides of it, so the workgroup reads its own block's
-// particles (one contiguous range of the cell-sorted records) plus the
-// particles of the lower-neighbour boundary cells (92 ranges in total) and
-// accumulates only the nodes it owns.  Every node is therefore written
-// exactly once with a plain store: no zero-fill and no global atomics.
-// Node sums are 64-bit fixed point (weights * 2^36) in LDS, so they are
-// exact sums of the float32 weights and independent of particle order.
+// Deposit (main file:1077-1265).  One workgroup owns the (8*TB*R)^3 nodes of
+// a tile of TB^3 8^3-cell blocks (R = ng/nx in {1, 2}).  A node receives
+// mass from the cells on both sides of it, so the workgroup reads its own
+// blocks' particles (TB^2 contiguous runs of the cell-sorted records) plus
+// the particles of the lower-neighbour boundary cells, and accumulates only
+// the nodes it owns.  Every node is written exactly once with a plain
+// store: no zero-fill and no global atomics.
+//
+// Node sums are 64-bit fixed point (weights * 2^36) in LDS, exact sums of
+// the float32 weights and independent of particle order.  A particle's
+// offset in its CIC cell is exact: the record holds u in [0, 8) relative to
+// its 8^3 block, and R*u - floor(R*u) is exact for R in {1, 2}.  Each lane
+// takes K consecutive particles of the sorted list and keeps the 8 node sums
+// in registers while they share the same lower node (dense cells), adding
+// them to LDS only when it changes.  Tiles are dispatched densest first, so
+// that the long tiles of halo cores overlap the rest.
 #include "psod_internal.h"
+
+#include <algorithm>
 
 int psod_fill_i32(psod_ctx *c, int32_t *p, int64_t n, int32_t v);
 int psod_fill_u64(psod_ctx *c, uint64_t *p, int64_t n, uint64_t v);
 
-constexpr int NSEG = 92;
 // weights in 2^-36 units: a node holds up to 2^28 unit weights
 constexpr float FIX_SCALE = 68719476736.0f;  // 2^36
 constexpr double FIX_INV = 1.0 / 68719476736.0;
-
-// Segment s of the ranges a block's nodes depend on: its own 512 cells and
-// the boundary cells of the 7 lower neighbour blocks.  Returns the local
-// cell key inside the (possibly neighbour) block, the run length in cells
-// and the block offset (dx, dy, dz) in {-1, 0}.
-__device__ inline void seg_decode(int s, int *loc, int *len, int *dx, int *dy,
-                                  int *dz) {
-  *dx = *dy = *dz = 0;
-  if (s == 0) { *loc = 0; *len = 512; return; }
-  if (s < 65) {  // x-1 face: lx=7, every (ly, lz)
-    const int t = s - 1;
-    *loc = ((t >> 3) * 8 + (t & 7)) * 8 + 7; *len = 1; *dx = -1; return;
-  }
-  if (s < 73) {  // y-1 face: ly=7, rows of 8 cells for each lz
-    *loc = ((s - 65) * 8 + 7) * 8; *len = 8; *dy = -1; return;
-  }
-  if (s == 73) { *loc = 7 * 64; *len = 64; *dz = -1; return; }  // z-1 face
-  if (s < 82) {  // x-1, y-1 edge
-    *loc = ((s - 74) * 8 + 7) * 8 + 7; *len = 1; *dx = -1; *dy = -1; return;
-  }
-  if (s < 90) {  // x-1, z-1 edge
-    *loc = (7 * 8 + (s - 82)) * 8 + 7; *len = 1; *dx = -1; *dz = -1; return;
-  }
-  if (s == 90) { *loc = (7 * 8 + 7) * 8; *len = 8; *dy = -1; *dz = -1; return; }
-  *loc = 511; *len = 1; *dx = -1; *dy = -1; *dz = -1;  // corner
-}
-
-struct SegTable {
-  int32_t start[NSEG];   // first sorted slot
-  int32_t pre[NSEG + 1]; // prefix of particle counts
-  int8_t off[NSEG][3];
-};
-
-__device__ inline void build_segments(SegTable &T, const int32_t *cell_start,
-                                      int32_t nb, int bx, int by, int bz) {
-  const int t = threadIdx.x;
-  int32_t cnt = 0;
-  if (t < NSEG) {
-    int loc, len, dx, dy, dz;
-    seg_decode(t, &loc, &len, &dx, &dy, &dz);
-    const int32_t nbk = (wrapi(bz + dz, nb) * nb + wrapi(by + dy, nb)) * nb +
-                        wrapi(bx + dx, nb);
-    const int32_t k0 = nbk * BS3 + loc;
-    const int32_t s0 = cell_start[k0];
-    cnt = cell_start[k0 + len] - s0;
-    T.start[t] = s0;
-    T.off[t][0] = (int8_t)dx;
-    T.off[t][1] = (int8_t)dy;
-    T.off[t][2] = (int8_t)dz;
-  }
-  // exclusive prefix over the first 128 threads (2 waves)
-  __shared__ int32_t carry;
-  if (t < 64) {
-    int32_t inc = wave_incl_scan(cnt);
-    if (t < NSEG) T.pre[t] = inc - cnt;
-    if (t == 63) carry = inc;
-  }
-  __syncthreads();
-  if (t >= 64 && t < 128) {
-    int32_t inc = wave_incl_scan(cnt);
-    if (t < NSEG) T.pre[t] = carry + inc - cnt;
-    if (t == NSEG - 1) T.pre[NSEG] = carry + inc;
-  }
-  __syncthreads();
-}
-
-__device__ inline int find_seg(const SegTable &T, int32_t j) {
-  int lo = 0, hi = NSEG;  // last s with pre[s] <= j
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (T.pre[mid] <= j) lo = mid; else hi = mid;
-  }
-  return lo;
-}
 
 // Tile of TB^3 blocks (8*TB cells per axis), R nodes per cell per axis.
 // Halo ranges: the boundary cells of the lower-neighbour blocks of the tile,
@@ -148,14 +82,66 @@ __device__ inline void tile_seg(int s, int *bi, int *bj, int *bk, int *loc,
   *loc = 511; *len = 1;
 }
 
+constexpr int DEP_K = 4;  // consecutive particles per lane
+constexpr int32_t DEP_CAPD = 8192;  // dense tiles dispatched first, at most
+
+// Dense tiles (at least DENSE_TILE own particles) are listed so that they
+// are dispatched first: their long runs then overlap the rest instead of
+// trailing it.  listed[tile] = 1 for a tile in dlist.
+constexpr int32_t DENSE_TILE = 1 << 14;
+__global__ void k_tile_dense(const int32_t *__restrict__ cell_start,
+                             int32_t nb, int32_t TB, int32_t ntile, int32_t capd,
+                             int32_t *__restrict__ listed,
+                             int32_t *__restrict__ dcount,
+                             int32_t *__restrict__ dlist) {
+  const int32_t tile = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tile >= ntile) return;
+  const int32_t nt = nb / TB;
+  const int tx = tile % nt, ty = (tile / nt) % nt, tz = tile / (nt * nt);
+  int32_t n = 0;
+  for (int k = 0; k < TB; k++)
+    for (int j = 0; j < TB; j++) {
+      const int32_t b0 = ((tz * TB + k) * nb + (ty * TB + j)) * nb + tx * TB;
+      n += cell_start[(b0 + TB) * BS3] - cell_start[b0 * BS3];
+    }
+  int32_t f = 0;
+  if (n >= DENSE_TILE) {
+    const int32_t k = atomicAdd(dcount, 1);
+    if (k < capd) {
+      dlist[k] = tile;
+      f = 1;
+    }
+  }
+  listed[tile] = f;
+}
+
+// Tile of workgroup b: b < capd takes the b-th listed dense tile; the others
+// take the remaining tiles in an XCD-aware order (workgroups go round-robin
+// over the 8 XCDs, each with its own L2: XCD x sweeps a contiguous slab of
+// tiles, so that a tile finds its neighbours' boundary cells in its L2).
+__device__ inline int32_t tile_of_block(int32_t b, int32_t ntile, int32_t capd,
+                                        const int32_t *dcount,
+                                        const int32_t *dlist,
+                                        const int32_t *listed) {
+  if (b < capd) return b < min(*dcount, capd) ? dlist[b] : -1;
+  const int32_t q = b - capd;
+  const int32_t tile = ntile % 8 == 0 ? (q % 8) * (ntile / 8) + q / 8 : q;
+  return (capd > 0 && listed[tile]) ? -1 : tile;
+}
+
 // Deposit of the (8*TB*R)^3 nodes owned by one tile.  Own particles come
 // from TB^2 contiguous runs of TB blocks; halo particles from the ranges of
-// tile_seg(), found by binary search on their prefix counts.
+// tile_seg(), found by binary search on their prefix counts.  The loop over
+// particles has a wave-uniform trip count (the segmented scan needs every
+// lane).
 template <int R, int TB>
 __global__ void __launch_bounds__(256)
 k_cic_deposit(const int32_t *__restrict__ cell_start,
               const float4 *__restrict__ rec, int32_t nx, int32_t ng,
-              int32_t shift, double norm, float *__restrict__ grid) {
+              int32_t shift, double norm, int32_t capd,
+              const int32_t *__restrict__ dcount,
+              const int32_t *__restrict__ dlist,
+              const int32_t *__restrict__ listed, float *__restrict__ grid) {
   constexpr int NN = 8 * TB * R;  // owned nodes per axis
   constexpr int NS = TileSegs<TB>::N;
   __shared__ unsigned long long acc[NN * NN * NN];
@@ -163,18 +149,24 @@ k_cic_deposit(const int32_t *__restrict__ cell_start,
   __shared__ int32_t hpre[NS + 1];
   __shared__ int8_t hoff[NS][3];
   __shared__ int32_t ostart[TB * TB], opre[TB * TB + 1];
+  __shared__ int32_t obnd[TB * TB][TB];  // own-list position of each block
   __shared__ int32_t wtot[4];
   const int32_t nb = nx >> 3;
   const int32_t nt = nb / TB;  // tiles per axis
-  const int32_t tile = blockIdx.x;
+  const int32_t tile = tile_of_block(blockIdx.x, nt * nt * nt, capd, dcount,
+                                     dlist, listed);
+  if (tile < 0) return;  // uniform per workgroup: before any barrier
   const int tx = tile % nt, ty = (tile / nt) % nt, tz = tile / (nt * nt);
   const int bx0 = tx * TB, by0 = ty * TB, bz0 = tz * TB;
+  const int l = threadIdx.x & 63;
   for (int i = threadIdx.x; i < NN * NN * NN; i += 256) acc[i] = 0ull;
   // own runs: blocks (bx0..bx0+TB-1, by0+j, bz0+k) are consecutive keys
   if (threadIdx.x < TB * TB) {
     const int j = threadIdx.x % TB, k = threadIdx.x / TB;
     const int32_t b0 = ((bz0 + k) * nb + (by0 + j)) * nb + bx0;
     ostart[threadIdx.x] = cell_start[b0 * BS3];
+    for (int q = 0; q < TB; q++)
+      obnd[threadIdx.x][q] = cell_start[(b0 + q) * BS3];
     opre[threadIdx.x + 1] = cell_start[(b0 + TB) * BS3];
   }
   // halo ranges and their prefix counts (blocks of 256 ranges)
@@ -196,7 +188,7 @@ k_cic_deposit(const int32_t *__restrict__ cell_start,
       hoff[s][1] = (int8_t)bj;
       hoff[s][2] = (int8_t)bk;
     }
-    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
     const int32_t inc = wave_incl_scan(cnt);
     if (l == 63) wtot[w] = inc;
     __syncthreads();
@@ -209,67 +201,86 @@ k_cic_deposit(const int32_t *__restrict__ cell_start,
   if (threadIdx.x == 0) {
     hpre[NS] = carry;
     opre[0] = 0;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    // prefix of the own runs (their ends were stored in opre[1..])
+    // prefix of the own runs (their ends were stored in opre[1..]) and the
+    // own-list position of every block
     int32_t p = 0;
     for (int r = 0; r < TB * TB; r++) {
       const int32_t n = opre[r + 1] - ostart[r];
+      for (int q = 0; q < TB; q++) obnd[r][q] = p + (obnd[r][q] - ostart[r]);
       opre[r + 1] = p + n;
       p += n;
     }
   }
   __syncthreads();
-  auto deposit = [&](const float4 &r, float ox, float oy, float oz) {
-    // position in CIC cells relative to the tile's first node
-    const float qx = (float)R * (r.x + ox), qy = (float)R * (r.y + oy),
-                qz = (float)R * (r.z + oz);
-    const float fx0 = floorf(qx), fy0 = floorf(qy), fz0 = floorf(qz);
-    const int ix = (int)fx0, iy = (int)fy0, iz = (int)fz0;
-    const float fx = qx - fx0, fy = qy - fy0, fz = qz - fz0;
-    const float wx[2] = {1.0f - fx, fx}, wy[2] = {1.0f - fy, fy},
-                wz[2] = {1.0f - fz, fz};
+  const int32_t nown = opre[TB * TB];
+  const int32_t ntot = nown + hpre[NS];
+  // register sums of the current lower node and their flush to LDS
+  unsigned long long ra[8];
+  int32_t rid = -1, rix = 0, riy = 0, riz = 0;
+  auto flush = [&]() {
+    if (rid < 0) return;
 #pragma unroll
     for (int c = 0; c < 8; c++) {
-      const int ax = ix + (c & 1), ay = iy + ((c >> 1) & 1), az = iz + (c >> 2);
+      const int ax = rix + (c & 1), ay = riy + ((c >> 1) & 1), az = riz + (c >> 2);
       if ((unsigned)ax < (unsigned)NN && (unsigned)ay < (unsigned)NN &&
-          (unsigned)az < (unsigned)NN) {
-        const float w = wx[c & 1] * wy[(c >> 1) & 1] * wz[c >> 2];
-        atomicAdd(&acc[(az * NN + ay) * NN + ax], __float2ull_rn(w * FIX_SCALE));
-      }
+          (unsigned)az < (unsigned)NN)
+        atomicAdd(&acc[(az * NN + ay) * NN + ax], ra[c]);
     }
   };
-  // own particles
-  const int32_t nown = opre[TB * TB];
-  for (int32_t t = threadIdx.x; t < nown; t += 256) {
-    int r = 0;
-    while (r + 1 < TB * TB && opre[r + 1] <= t) r++;
-    const int32_t k = ostart[r] + (t - opre[r]);
-    const float4 rv = rec[k];
-    // block of this record inside the tile: y/z from the run index, x from
-    // the block boundaries inside the run
-    const int j = r % TB, kk = r / TB;
-    int i = 0;
-    if (TB > 1) {
-      const int32_t b0 = ((bz0 + kk) * nb + (by0 + j)) * nb + bx0;
-      for (int q = 1; q < TB; q++)
-        if (k >= cell_start[(b0 + q) * BS3]) i = q;
+  // lane l of wave w takes list positions base + w*64*K + l*K + j, j < K
+  const int wv = threadIdx.x >> 6;
+  for (int32_t base = 0; base < ntot; base += 256 * DEP_K) {
+    for (int j = 0; j < DEP_K; j++) {
+      const int32_t t = base + wv * 64 * DEP_K + l * DEP_K + j;
+      if (t >= ntot) break;
+      int32_t k = 0, bxo = 0, byo = 0, bzo = 0;
+      if (t < nown) {
+        int r = 0;
+        while (r + 1 < TB * TB && opre[r + 1] <= t) r++;
+        k = ostart[r] + (t - opre[r]);
+        byo = r % TB;
+        bzo = r / TB;
+        for (int q = 1; q < TB; q++)
+          if (t >= obnd[r][q]) bxo = q;
+      } else {
+        const int32_t th = t - nown;
+        int lo = 0, hi = NS;  // last s with hpre[s] <= th
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (hpre[mid] <= th) lo = mid; else hi = mid;
+        }
+        k = hstart[lo] + (th - hpre[lo]);
+        bxo = hoff[lo][0];
+        byo = hoff[lo][1];
+        bzo = hoff[lo][2];
+      }
+      // lower node (tile-relative) and exact offsets
+      const float4 r = rec[k];
+      const float qx = (float)R * r.x, qy = (float)R * r.y, qz = (float)R * r.z;
+      const float fx0 = floorf(qx), fy0 = floorf(qy), fz0 = floorf(qz);
+      const int32_t ix = 8 * R * bxo + (int32_t)fx0;
+      const int32_t iy = 8 * R * byo + (int32_t)fy0;
+      const int32_t iz = 8 * R * bzo + (int32_t)fz0;
+      const float fx = qx - fx0, fy = qy - fy0, fz = qz - fz0;
+      const float wx[2] = {1.0f - fx, fx}, wy[2] = {1.0f - fy, fy},
+                  wz[2] = {1.0f - fz, fz};
+      const int32_t id = ((iz + 16) * 64 + (iy + 16)) * 64 + (ix + 16);
+      if (id != rid) {
+        flush();
+        rid = id;
+        rix = ix;
+        riy = iy;
+        riz = iz;
+#pragma unroll
+        for (int c = 0; c < 8; c++) ra[c] = 0ull;
+      }
+#pragma unroll
+      for (int c = 0; c < 8; c++)
+        ra[c] += __float2ull_rn(wx[c & 1] * wy[(c >> 1) & 1] * wz[c >> 2] *
+                                FIX_SCALE);
     }
-    deposit(rv, 8.0f * (float)i, 8.0f * (float)j, 8.0f * (float)kk);
   }
-  // halo particles
-  const int32_t nh = hpre[NS];
-  for (int32_t t = threadIdx.x; t < nh; t += 256) {
-    int lo = 0, hi = NS;  // last s with hpre[s] <= t
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (hpre[mid] <= t) lo = mid; else hi = mid;
-    }
-    const float4 rv = rec[hstart[lo] + (t - hpre[lo])];
-    deposit(rv, 8.0f * (float)hoff[lo][0], 8.0f * (float)hoff[lo][1],
-            8.0f * (float)hoff[lo][2]);
-  }
+  flush();
   __syncthreads();
   const int32_t gx0 = bx0 * 8 * R, gy0 = by0 * 8 * R, gz0 = bz0 * 8 * R;
   for (int i = threadIdx.x; i < NN * NN * NN; i += 256) {
@@ -383,17 +394,33 @@ extern "C" int psod_cic_deposit(psod_ctx *ctx, int64_t npart, int32_t nx,
     return psod_set_error(PSOD_E_ARG, "psod_cic_deposit: bad arguments");
   const int32_t nb = nx / 8;
   const double norm = (double)ng * (double)ng * (double)ng / (double)npart;
-  if (ng == nx && nb % 2 == 0) {
-    const unsigned ntile = (unsigned)(nb / 2) * (nb / 2) * (nb / 2);
-    k_cic_deposit<1, 2><<<ntile, 256, 0, ctx->stream>>>(
-        d_cell_start, (const float4 *)d_rec, nx, ng, shift, norm, d_grid);
-  } else if (ng == nx) {
-    k_cic_deposit<1, 1><<<(unsigned)nb * nb * nb, 256, 0, ctx->stream>>>(
-        d_cell_start, (const float4 *)d_rec, nx, ng, shift, norm, d_grid);
-  } else {
-    k_cic_deposit<2, 1><<<(unsigned)nb * nb * nb, 256, 0, ctx->stream>>>(
-        d_cell_start, (const float4 *)d_rec, nx, ng, shift, norm, d_grid);
+  const int32_t TB = (ng == nx && nb % 2 == 0) ? 2 : 1;
+  const int32_t ntile = (nb / TB) * (nb / TB) * (nb / TB);
+  const int32_t capd = std::min<int32_t>(ntile, DEP_CAPD);
+  int32_t *listed = (int32_t *)psod_scratch(
+      ctx, S_TILE, (size_t)(ntile + capd + 64) * 4);
+  if (!listed) return PSOD_E_NOMEM;
+  int32_t *dcount = listed + ntile, *dlist = dcount + 32;
+  hipStream_t st = ctx->stream;
+  const unsigned nblk = (unsigned)(ntile + capd);
+  if (capd > 0) {
+    PSOD_TRY(psod_fill_i32(ctx, dcount, 1, 0));
+    k_tile_dense<<<grid_for(ntile, 256), 256, 0, st>>>(
+        d_cell_start, nb, TB, ntile, capd, listed, dcount, dlist);
+    PSOD_LAUNCH_CHECK();
   }
+  if (ng == nx && TB == 2)
+    k_cic_deposit<1, 2><<<nblk, 256, 0, st>>>(
+        d_cell_start, (const float4 *)d_rec, nx, ng, shift, norm, capd, dcount,
+        dlist, listed, d_grid);
+  else if (ng == nx)
+    k_cic_deposit<1, 1><<<nblk, 256, 0, st>>>(
+        d_cell_start, (const float4 *)d_rec, nx, ng, shift, norm, capd, dcount,
+        dlist, listed, d_grid);
+  else
+    k_cic_deposit<2, 1><<<nblk, 256, 0, st>>>(
+        d_cell_start, (const float4 *)d_rec, nx, ng, shift, norm, capd, dcount,
+        dlist, listed, d_grid);
   PSOD_LAUNCH_CHECK();
   PSOD_HIP(hipStreamSynchronize(ctx->stream));
   return PSOD_OK;

@@ -58,6 +58,7 @@ enum ScratchSlot {
   S_GREEDY,
   S_GREEDY2,
   S_GREEDY3,
+  S_TILE,      // deposit tile counts and dispatch order
   S_NSLOT
 };
 
