@@ -131,9 +131,7 @@ __device__ inline int32_t tile_of_block(int32_t b, int32_t ntile, int32_t capd,
 
 // Deposit of the (8*TB*R)^3 nodes owned by one tile.  Own particles come
 // from TB^2 contiguous runs of TB blocks; halo particles from the ranges of
-// tile_seg(), found by binary search on their prefix counts.  The loop over
-// particles has a wave-uniform trip count (the segmented scan needs every
-// lane).
+// tile_seg(), found by binary search on their prefix counts.
 template <int R, int TB>
 __global__ void __launch_bounds__(256)
 k_cic_deposit(const int32_t *__restrict__ cell_start,
