@@ -172,6 +172,7 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_step = wall / args.steps * 1e3
+    cs = ctx.sod_stats()  # execution paths of the last timed step
     st = {n: float(np.mean([a.elapsed_time(b) for a, b in ev[n]]))
           for n in names}
     status = out["status"].cpu().numpy()
@@ -250,8 +251,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 split (int cell + f32 offset), f64 radii/COM, "
-                 "i64 fixed-point sums",
+        "dtype": "f32",
+        "precision": "f32 positions split as int cell + f32 offset; f64 "
+                     "radii, overdensities and COM; i64 fixed-point sums "
+                     "(CIC weights, centre-search terms)",
         "data": "synthetic (seeded NFW clumps on a uniform background, "
                 "generated on the device)",
         "config": {"workload": f"{args.config}: {N} particles ({nx}^3), "
@@ -263,6 +266,14 @@ def main():
         "sod_stage_halos_per_s": round(nh / (st["sod"] * 1e-3), 1),
         "stages": stages,
         "roofline": roofline,
+        "centre_search": dict(
+            pair_tests=cs["pair_tests"], ms=round(cs["centre_ms"], 3),
+            pair_tests_per_s=(round(cs["pair_tests"] / (cs["centre_ms"] * 1e-3), 1)
+                              if cs["centre_ms"] > 0 else None),
+            tasks=cs["centre_tasks_all"], evaluated=cs["centre_tasks"],
+            skipped_by_bound=cs["centre_skipped"]),
+        "sod_paths": dict(wave=cs["wave"], workgroup=cs["workgroup"],
+                          rerun=cs["retried"]),
         "plain_fp64": plain,
         "cpu_baseline": cpu,
     }

@@ -229,10 +229,13 @@ class Context:
     def sod_stats(self):
         """Execution-path counts of the last sod_fof call: halos grown by
         one wave, by one workgroup, re-run with larger collection slices,
-        and the keys per slice of that re-run."""
-        a = (C.c_int64 * 4)()
-        _lib.check(_L.psod_sod_stats(self.h, a))
-        return dict(wave=a[0], workgroup=a[1], retried=a[2], retry_slice=a[3])
+        the keys per slice of that re-run, and the centre search's tasks
+        (evaluated, skipped by the bound, in all) and pair tests."""
+        a = (C.c_int64 * 9)()
+        _lib.check(_L.psod_sod_stats(self.h, a, 9))
+        return dict(wave=a[0], workgroup=a[1], retried=a[2], retry_slice=a[3],
+                    centre_tasks=a[4], centre_skipped=a[5], pair_tests=a[6],
+                    centre_tasks_all=a[7], centre_ms=a[8] * 1e-6)
 
     # ---- host drop-ins ---------------------------------------------------
     def host_cic(self, pos, ng, mass=None, xxmin=0.0):

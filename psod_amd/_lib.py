@@ -102,7 +102,7 @@ def load():
         "psod_synchronize": (C.c_int, [v]),
         "psod_stream": (v, [v]),
         "psod_scratch_bytes": (i64, [v]),
-        "psod_sod_stats": (C.c_int, [v, P(i64)]),
+        "psod_sod_stats": (C.c_int, [v, P(i64), i32]),
         "psod_cell_sort": (C.c_int, [v, i64, v, v, i32, v, v, v, v]),
         "psod_cic_deposit": (C.c_int, [v, i64, i32, v, v, i32, i32, v]),
         "psod_cic_interp": (C.c_int, [v, i64, i32, v, v, i32, i32, v, v, v,

@@ -38,6 +38,12 @@ extern "C" int psod_create(int device, psod_ctx **out) {
     return psod_set_error(PSOD_E_DEVICE, "hipStreamCreate: %s",
                           hipGetErrorString(es));
   }
+  if (hipEventCreate(&c->ev[0]) != hipSuccess ||
+      hipEventCreate(&c->ev[1]) != hipSuccess) {
+    hipStreamDestroy(c->stream);
+    free(c);
+    return psod_set_error(PSOD_E_DEVICE, "hipEventCreate failed");
+  }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess)
     c->num_cus = prop.multiProcessorCount;
@@ -53,6 +59,8 @@ extern "C" int psod_destroy(psod_ctx *c) {
   hipStreamSynchronize(c->stream);
   for (int s = 0; s < S_NSLOT; s++)
     if (c->slot[s]) hipFree(c->slot[s]);
+  hipEventDestroy(c->ev[0]);
+  hipEventDestroy(c->ev[1]);
   hipStreamDestroy(c->stream);
   free(c);
   return PSOD_OK;
@@ -73,9 +81,9 @@ extern "C" int64_t psod_scratch_bytes(psod_ctx *c) {
   return s;
 }
 
-extern "C" int psod_sod_stats(psod_ctx *c, int64_t out[4]) {
-  if (!c || !out) return psod_set_error(PSOD_E_ARG, "psod_sod_stats: null argument");
-  for (int i = 0; i < 4; i++) out[i] = c->sod_stat[i];
+extern "C" int psod_sod_stats(psod_ctx *c, int64_t *out, int32_t n) {
+  if (!c || !out || n < 0) return psod_set_error(PSOD_E_ARG, "psod_sod_stats: bad arguments");
+  for (int i = 0; i < n && i < 9; i++) out[i] = c->sod_stat[i];
   return PSOD_OK;
 }
 

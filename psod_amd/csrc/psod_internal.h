@@ -68,7 +68,8 @@ struct psod_ctx {
   void *slot[S_NSLOT];
   size_t cap[S_NSLOT];
   int num_cus;
-  int64_t sod_stat[4];  // see psod_sod_stats
+  int64_t sod_stat[9];  // see psod_sod_stats
+  hipEvent_t ev[2];     // brackets the centre search (sod_stat[8])
 };
 
 // Returns a device buffer of at least `bytes` in `slot` (contents not kept

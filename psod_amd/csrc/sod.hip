@@ -329,11 +329,18 @@ k_sod_small(SodCfg c, const int32_t *__restrict__ cell_start,
       const int j = S.a.cidx[ci];
       const float4 p = S.a.nb[j];
       uint64_t acc = 0;
-      for (int32_t t = 0; t < n5; t++) {
-        const float4 q = S.a.nb[t];
-        const float dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
-        const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-        if (d2 <= c.rs2f) acc += sterm_fix(d2, c.eps2f);
+      // u32 partial sums of at most 256 terms of at most 2^24
+      for (int32_t t0 = 0; t0 < n5; t0 += 256) {
+        const int32_t t1 = min(n5, t0 + 256);
+        uint32_t part = 0;
+        for (int32_t t = t0; t < t1; t++) {
+          const float4 q = S.a.nb[t];
+          const float dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
+          const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+          const uint32_t term = sterm_fix(d2, c.eps2f);
+          part += d2 <= c.rs2f ? term : 0u;
+        }
+        acc += part;
       }
       const uint64_t key = ((uint64_t)(uint32_t)S.a.rank[j] << 32) |
                            (uint32_t)__float_as_int(p.w);
@@ -628,25 +635,45 @@ __device__ inline uint32_t sterm_bound(float g2, const SodCfg &c) {
 // neighbour chunks are compact; a chunk whose box lies beyond the reach of
 // the candidates' box is skipped.
 //
-// Branch and bound, exact: phase 0 runs the tasks of the seed cell (which
-// holds the density peak) and folds their best S into hbest[halo]; phase 1
-// runs the other tasks, first bounding each candidate's S by the chunk
-// counts times the term at the candidate-to-chunk-box distance, and skips
-// the task when no bound reaches hbest (such a task can neither hold the
-// maximum nor tie with it).
+// Branch and bound, exact, in three launches.  Phase 0 bounds every task:
+// each candidate's S is at most the sum over groups of 8 neighbours of the
+// group count times the term at the candidate-to-group-box distance; the
+// task's bound (ubt) and each halo's largest bound (hpil) are stored.
+// Phase 1 evaluates the tasks holding their halo's largest bound exactly and
+// folds their S into hbest.  Phase 2 evaluates the other tasks whose bound
+// reaches hbest and skips the rest: a skipped task can neither hold the
+// maximum nor tie with it.
 __global__ void __launch_bounds__(256)
 k_centre_task(SodCfg c, const int32_t *__restrict__ cell_start,
               const float4 *__restrict__ rec, const HaloPlan *__restrict__ plan,
               const int32_t *__restrict__ large, const int4 *__restrict__ tasks,
               const int32_t *__restrict__ ntask_total,
               uint64_t *__restrict__ taskres, int phase,
-              unsigned long long *__restrict__ hbest) {
+              unsigned long long *__restrict__ hbest,
+              unsigned long long *__restrict__ hpil,
+              unsigned long long *__restrict__ ubt,
+              unsigned long long *__restrict__ cstat) {
   const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int32_t nt = *ntask_total;
   const float cut = c.rs2f * 1.0001f + 1e-6f;  // conservative chunk culling
   for (int32_t t = blockIdx.x * 4 + w; t < nt; t += gridDim.x * 4) {
     const int4 tk = tasks[t];
-    if ((tk.y == 13) != (phase == 0)) continue;  // 13: the seed cell
+    if (phase > 0) {
+      const unsigned long long ubv = ubt[t], pil = hpil[tk.x];
+      if ((phase == 1) != (ubv == pil)) continue;
+      if (phase == 2) {
+        const unsigned long long best = __hip_atomic_load(
+            &hbest[tk.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (ubv < best) {
+          if (l == 0) {
+            taskres[2 * (int64_t)t] = 0;
+            taskres[2 * (int64_t)t + 1] = ~0ull;
+            atomicAdd(&cstat[1], 1ull);
+          }
+          continue;
+        }
+      }
+    }
     const HaloPlan pl = plan[large[tk.x]];
     int ci, cj, ck;
     walk_off(tk.y, 1, &ci, &cj, &ck);
@@ -669,8 +696,37 @@ k_centre_task(SodCfg c, const int32_t *__restrict__ cell_start,
     cb1[0] = wave_maxf(have ? px : -INFINITY);
     cb1[1] = wave_maxf(have ? py : -INFINITY);
     cb1[2] = wave_maxf(have ? pz : -INFINITY);
-    if (phase == 1) {
-      uint64_t ub = 0;
+    if (phase == 0) {
+      // boxes of the 8 groups of 8 candidates (uniform), then every
+      // neighbour (one per lane) adds the term at its distance to each box:
+      // ubg[g] bounds S of every candidate of group g
+      float g0x = have ? px : INFINITY, g0y = have ? py : INFINITY,
+            g0z = have ? pz : INFINITY;
+      float g1x = have ? px : -INFINITY, g1y = have ? py : -INFINITY,
+            g1z = have ? pz : -INFINITY;
+#pragma unroll
+      for (int o = 1; o < 8; o <<= 1) {
+        g0x = fminf(g0x, __shfl_xor(g0x, o, 64));
+        g0y = fminf(g0y, __shfl_xor(g0y, o, 64));
+        g0z = fminf(g0z, __shfl_xor(g0z, o, 64));
+        g1x = fmaxf(g1x, __shfl_xor(g1x, o, 64));
+        g1y = fmaxf(g1y, __shfl_xor(g1y, o, 64));
+        g1z = fmaxf(g1z, __shfl_xor(g1z, o, 64));
+      }
+      float B[8][6];
+#pragma unroll
+      for (int g = 0; g < 8; g++) {
+        B[g][0] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g0x), 8 * g));
+        B[g][1] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g0y), 8 * g));
+        B[g][2] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g0z), 8 * g));
+        B[g][3] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g1x), 8 * g));
+        B[g][4] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g1y), 8 * g));
+        B[g][5] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g1z), 8 * g));
+      }
+      const int ngr = (tk.w + 7) >> 3;  // groups holding candidates
+      uint64_t ubg[8];
+#pragma unroll
+      for (int g = 0; g < 8; g++) ubg[g] = 0;
       for (int q = 0; q < 27; q++) {
         int ei, ej, ek;
         walk_off(q, 1, &ei, &ej, &ek);
@@ -684,43 +740,36 @@ k_centre_task(SodCfg c, const int32_t *__restrict__ cell_start,
         int32_t ln[3];
         const int32_t key = key_at(c.nx, pl.cs, di, dj, dk, ln);
         const int32_t a = cell_start[key], b = cell_start[key + 1];
-        for (int32_t s = a; s < b; s += 64) {
-          const int32_t m = min(64, b - s);
-          float vx = 0.f, vy = 0.f, vz = 0.f;
-          if (l < m) {
-            const float4 r = rec[s + l];
-            vx = rel1(r.x, ln[0], di, 0.0f);
-            vy = rel1(r.y, ln[1], dj, 0.0f);
-            vz = rel1(r.z, ln[2], dk, 0.0f);
+        for (int32_t s = a + l; s < b; s += 64) {
+          const float4 r = rec[s];
+          const float vx = rel1(r.x, ln[0], di, 0.0f);
+          const float vy = rel1(r.y, ln[1], dj, 0.0f);
+          const float vz = rel1(r.z, ln[2], dk, 0.0f);
+#pragma unroll
+          for (int g = 0; g < 8; g++) {
+            if (g < ngr) {
+              const float gx = fmaxf(0.0f, fmaxf(B[g][0] - vx, vx - B[g][3]));
+              const float gy = fmaxf(0.0f, fmaxf(B[g][1] - vy, vy - B[g][4]));
+              const float gz = fmaxf(0.0f, fmaxf(B[g][2] - vz, vz - B[g][5]));
+              ubg[g] += sterm_bound(gx * gx + gy * gy + gz * gz, c);
+            }
           }
-          const float n0x = wave_minf(l < m ? vx : INFINITY);
-          const float n0y = wave_minf(l < m ? vy : INFINITY);
-          const float n0z = wave_minf(l < m ? vz : INFINITY);
-          const float n1x = wave_maxf(l < m ? vx : -INFINITY);
-          const float n1y = wave_maxf(l < m ? vy : -INFINITY);
-          const float n1z = wave_maxf(l < m ? vz : -INFINITY);
-          const float gx = fmaxf(0.0f, fmaxf(n0x - px, px - n1x));
-          const float gy = fmaxf(0.0f, fmaxf(n0y - py, py - n1y));
-          const float gz = fmaxf(0.0f, fmaxf(n0z - pz, pz - n1z));
-          ub += (uint64_t)m * sterm_bound(gx * gx + gy * gy + gz * gz, c);
         }
       }
-      uint64_t ubmax = have ? ub : 0;
-      for (int o = 32; o > 0; o >>= 1) {
-        const uint64_t v = __shfl_xor(ubmax, o, 64);
-        ubmax = v > ubmax ? v : ubmax;
+      uint64_t ubmax = 0;
+#pragma unroll
+      for (int g = 0; g < 8; g++) {
+        const uint64_t v = wave_sum(ubg[g]);
+        if (g < ngr) ubmax = v > ubmax ? v : ubmax;
       }
-      const unsigned long long best =
-          __hip_atomic_load(&hbest[tk.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (ubmax < best) {
-        if (l == 0) {
-          taskres[2 * (int64_t)t] = 0;
-          taskres[2 * (int64_t)t + 1] = ~0ull;
-        }
-        continue;
+      if (l == 0) {
+        ubt[t] = ubmax;
+        atomicMax(&hpil[tk.x], (unsigned long long)ubmax);
       }
+      continue;
     }
     uint64_t acc = 0;
+    uint64_t npair = 0;  // neighbours tested per candidate
     for (int q = 0; q < 27; q++) {
       int ei, ej, ek;
       walk_off(q, 1, &ei, &ej, &ek);
@@ -753,7 +802,9 @@ k_centre_task(SodCfg c, const int32_t *__restrict__ cell_start,
           if (box_gap2(cb0, cb1, nb0, nb1) > cut) continue;
         }
         uint32_t part = 0;  // <= 64 terms of <= 2^24
-        for (int32_t u = 0; u < m; u++) {
+        // m is uniform: a scalar trip count, and a select instead of a branch
+        const int32_t mu = __builtin_amdgcn_readfirstlane(m);
+        for (int32_t u = 0; u < mu; u++) {
           const float qx = __int_as_float(
               __builtin_amdgcn_readlane(__float_as_int(vx), u));
           const float qy = __int_as_float(
@@ -762,9 +813,11 @@ k_centre_task(SodCfg c, const int32_t *__restrict__ cell_start,
               __builtin_amdgcn_readlane(__float_as_int(vz), u));
           const float dx = qx - px, dy = qy - py, dz = qz - pz;
           const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-          part += d2 <= c.rs2f ? sterm_fix(d2, c.eps2f) : 0u;
+          const uint32_t term = sterm_fix(d2, c.eps2f);
+          part += d2 <= c.rs2f ? term : 0u;
         }
         acc += part;
+        npair += (uint64_t)m;
       }
     }
     uint64_t key = have ? (((uint64_t)(uint32_t)tk.y << 32) | (uint32_t)pidx)
@@ -775,6 +828,8 @@ k_centre_task(SodCfg c, const int32_t *__restrict__ cell_start,
       taskres[2 * (int64_t)t] = acc;
       taskres[2 * (int64_t)t + 1] = key;
       atomicMax(&hbest[tk.x], (unsigned long long)acc);
+      atomicAdd(&cstat[0], 1ull);
+      atomicAdd(&cstat[2], (unsigned long long)(npair * (uint64_t)tk.w));
     }
   }
 }
@@ -1541,7 +1596,7 @@ extern "C" int psod_sod_fof(psod_ctx *ctx, const psod_sod_params *p,
   const int32_t nlarge = hc[1];
   ctx->sod_stat[0] = nhalo - nlarge;
   ctx->sod_stat[1] = nlarge;
-  ctx->sod_stat[2] = ctx->sod_stat[3] = 0;
+  for (int i = 2; i < 9; i++) ctx->sod_stat[i] = 0;
   if (nlarge == 0) return PSOD_OK;
   // largest halos first, so that the persistent growth workgroups do not
   // finish on a giant halo picked up last
@@ -1558,7 +1613,7 @@ extern "C" int psod_sod_fof(psod_ctx *ctx, const psod_sod_params *p,
   }
   // centre-search tasks of the large halos
   int32_t *ntask = (int32_t *)psod_scratch(
-      ctx, S_MISC, (size_t)(5 * nlarge + 128) * 4 + (size_t)nlarge * 8 + 256);
+      ctx, S_MISC, (size_t)(5 * nlarge + 128) * 4 + (size_t)nlarge * 16 + 512);
   if (!ntask) return PSOD_E_NOMEM;
   // best S found so far per large halo (branch and bound of the search)
   unsigned long long *hbest = (unsigned long long *)(
@@ -1576,23 +1631,36 @@ extern "C" int psod_sod_fof(psod_ctx *ctx, const psod_sod_params *p,
   PSOD_HIP(hipStreamSynchronize(st));
   if (nt > 0) {
     int4 *tasks = (int4 *)psod_scratch(ctx, S_TASKRES,
-                                       (size_t)nt * (sizeof(int4) + 16));
+                                       (size_t)nt * (sizeof(int4) + 24));
     if (!tasks) return PSOD_E_NOMEM;
     uint64_t *tres = (uint64_t *)(tasks + nt);
+    unsigned long long *ubt = (unsigned long long *)(tres + 2 * (int64_t)nt);
     k_task_emit<<<grid_for(nlarge, 256), 256, 0, st>>>(
         c, cat->cell_start, plan, large, nlarge, ntask, task0, tasks);
     PSOD_LAUNCH_CHECK();
-    PSOD_TRY(psod_fill_u64(ctx, (uint64_t *)hbest, nlarge, 0));
-    for (int phase = 0; phase < 2; phase++) {
+    // hbest[nlarge..nlarge+3]: tasks evaluated, tasks skipped, pair tests;
+    // then each halo's largest task bound
+    unsigned long long *cstat = hbest + nlarge;
+    unsigned long long *hpil = cstat + 4;
+    PSOD_TRY(psod_fill_u64(ctx, (uint64_t *)hbest, 2 * (int64_t)nlarge + 4, 0));
+    PSOD_HIP(hipEventRecord(ctx->ev[0], st));
+    for (int phase = 0; phase < 3; phase++) {
       k_centre_task<<<grid_for((int64_t)nt, 4, (int64_t)ctx->num_cus * 16),
                       256, 0, st>>>(c, cat->cell_start,
                                     (const float4 *)cat->rec, plan, large,
-                                    tasks, ntot, tres, phase, hbest);
+                                    tasks, ntot, tres, phase, hbest, hpil,
+                                    ubt, cstat);
       PSOD_LAUNCH_CHECK();
     }
     k_centre_reduce<<<grid_for(nlarge, 256), 256, 0, st>>>(plan, large,
                                                            nlarge, tres);
     PSOD_LAUNCH_CHECK();
+    PSOD_HIP(hipEventRecord(ctx->ev[1], st));
+    // read back with the growth's counters (the stream is synchronised
+    // before this call returns)
+    PSOD_HIP(hipMemcpyAsync(ctx->sod_stat + 4, cstat, 3 * 8,
+                            hipMemcpyDeviceToHost, st));
+    ctx->sod_stat[7] = nt;
   }
   const unsigned ngrow = (unsigned)min(nlarge, ctx->num_cus * 2);
   int32_t *hist = (int32_t *)psod_scratch(ctx, S_BINS,
@@ -1620,6 +1688,11 @@ extern "C" int psod_sod_fof(psod_ctx *ctx, const psod_sod_params *p,
   PSOD_HIP(hipMemcpyAsync(hf, counters + 4, 8, hipMemcpyDeviceToHost, st));
   PSOD_HIP(hipMemcpyAsync(&hneed, needmax, 8, hipMemcpyDeviceToHost, st));
   PSOD_HIP(hipStreamSynchronize(st));
+  if (ctx->sod_stat[7] > 0) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]) == hipSuccess)
+      ctx->sod_stat[8] = (int64_t)((double)ms * 1e6);
+  }
   // Further launches for the halos whose bands exceeded the slice: fewer
   // workgroups with larger slices.  A halo re-run from its first reach may
   // need more than its first failure showed (a larger reach later), so the

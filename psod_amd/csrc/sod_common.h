@@ -129,7 +129,9 @@ __device__ inline void centre_scaled(const float *pos, int32_t c, int32_t nx,
 constexpr float SFIX = 262144.0f;  // 2^18
 constexpr double SFIX_INV = 1.0 / 262144.0;
 __device__ inline uint32_t sterm_fix(float d2, float eps2) {
-  return (uint32_t)(rsqrtf(d2 + eps2) * SFIX + 0.5f);
+  // v_rsq_f32 without the denormal-input rescaling of rsqrtf: the argument
+  // is at least eps2, a normal float, so both give the same value
+  return (uint32_t)(__builtin_amdgcn_rsqf(d2 + eps2) * SFIX + 0.5f);
 }
 
 // per-halo plan
