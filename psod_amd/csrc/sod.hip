@@ -653,6 +653,7 @@ k_centre_task(SodCfg c, const int32_t *__restrict__ cell_start,
               unsigned long long *__restrict__ hpil,
               unsigned long long *__restrict__ ubt,
               unsigned long long *__restrict__ cstat) {
+  __shared__ float4 nbuf[4][64];  // one neighbour chunk per wave
   const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int32_t nt = *ntask_total;
   const float cut = c.rs2f * 1.0001f + 1e-6f;  // conservative chunk culling
@@ -802,22 +803,27 @@ k_centre_task(SodCfg c, const int32_t *__restrict__ cell_start,
           if (box_gap2(cb0, cb1, nb0, nb1) > cut) continue;
         }
         uint32_t part = 0;  // <= 64 terms of <= 2^24
+        // the chunk goes through this wave's LDS slot and is read back with
+        // wave-uniform addresses (broadcast reads, no lane moves)
+        nbuf[w][l] = make_float4(vx, vy, vz, 0.0f);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         // m is uniform: a scalar trip count, and a select instead of a branch
         const int32_t mu = __builtin_amdgcn_readfirstlane(m);
+#pragma unroll 4
         for (int32_t u = 0; u < mu; u++) {
-          const float qx = __int_as_float(
-              __builtin_amdgcn_readlane(__float_as_int(vx), u));
-          const float qy = __int_as_float(
-              __builtin_amdgcn_readlane(__float_as_int(vy), u));
-          const float qz = __int_as_float(
-              __builtin_amdgcn_readlane(__float_as_int(vz), u));
-          const float dx = qx - px, dy = qy - py, dz = qz - pz;
+          const float4 qv = nbuf[w][u];
+          const float dx = qv.x - px, dy = qv.y - py, dz = qv.z - pz;
           const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
           const uint32_t term = sterm_fix(d2, c.eps2f);
           part += d2 <= c.rs2f ? term : 0u;
         }
         acc += part;
         npair += (uint64_t)m;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       }
     }
     uint64_t key = have ? (((uint64_t)(uint32_t)tk.y << 32) | (uint32_t)pidx)
