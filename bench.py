@@ -271,7 +271,8 @@ def main():
             pair_tests_per_s=(round(cs["pair_tests"] / (cs["centre_ms"] * 1e-3), 1)
                               if cs["centre_ms"] > 0 else None),
             tasks=cs["centre_tasks_all"], evaluated=cs["centre_tasks"],
-            skipped_by_bound=cs["centre_skipped"]),
+            skipped_by_bound=cs["centre_skipped"],
+            abandoned=cs["centre_aborted"]),
         "sod_paths": dict(wave=cs["wave"], workgroup=cs["workgroup"],
                           rerun=cs["retried"]),
         "plain_fp64": plain,

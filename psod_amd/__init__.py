@@ -231,11 +231,12 @@ class Context:
         one wave, by one workgroup, re-run with larger collection slices,
         the keys per slice of that re-run, and the centre search's tasks
         (evaluated, skipped by the bound, in all) and pair tests."""
-        a = (C.c_int64 * 9)()
-        _lib.check(_L.psod_sod_stats(self.h, a, 9))
+        a = (C.c_int64 * 10)()
+        _lib.check(_L.psod_sod_stats(self.h, a, 10))
         return dict(wave=a[0], workgroup=a[1], retried=a[2], retry_slice=a[3],
                     centre_tasks=a[4], centre_skipped=a[5], pair_tests=a[6],
-                    centre_tasks_all=a[7], centre_ms=a[8] * 1e-6)
+                    centre_tasks_all=a[7], centre_ms=a[8] * 1e-6,
+                    centre_aborted=a[9])
 
     # ---- host drop-ins ---------------------------------------------------
     def host_cic(self, pos, ng, mass=None, xxmin=0.0):

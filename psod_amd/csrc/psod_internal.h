@@ -68,7 +68,7 @@ struct psod_ctx {
   void *slot[S_NSLOT];
   size_t cap[S_NSLOT];
   int num_cus;
-  int64_t sod_stat[9];  // see psod_sod_stats
+  int64_t sod_stat[10];  // see psod_sod_stats
   hipEvent_t ev[2];     // brackets the centre search (sod_stat[8])
 };
 

@@ -621,6 +621,12 @@ __device__ inline float box_gap2(const float *a0, const float *a1,
   return s;
 }
 
+// neighbour cells of a candidate cell, nearest first (walk_off indices:
+// the cell itself, 6 faces, 12 edges, 8 corners)
+__constant__ int8_t kCellOrder[27] = {13, 4,  10, 12, 14, 16, 22, 1,  3,
+                                      5,  7,  9,  11, 15, 17, 19, 21, 23,
+                                      25, 0,  2,  6,  8,  18, 20, 24, 26};
+
 // Upper bound of the term of any pair at squared distance >= g2 (the float
 // d2 of a pair is at least g2 up to a few ulps, hence the 2e-5 shrink; +4
 // covers non-monotone rounding of the reciprocal square root)
@@ -652,6 +658,7 @@ k_centre_task(SodCfg c, const int32_t *__restrict__ cell_start,
               unsigned long long *__restrict__ hbest,
               unsigned long long *__restrict__ hpil,
               unsigned long long *__restrict__ ubt,
+              unsigned long long *__restrict__ cub,
               unsigned long long *__restrict__ cstat) {
   __shared__ float4 nbuf[4][64];  // one neighbour chunk per wave
   const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -725,9 +732,10 @@ k_centre_task(SodCfg c, const int32_t *__restrict__ cell_start,
         B[g][5] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g1z), 8 * g));
       }
       const int ngr = (tk.w + 7) >> 3;  // groups holding candidates
-      uint64_t ubg[8];
+      uint64_t ubt8[8];  // the task's bound per group, all cells
 #pragma unroll
-      for (int g = 0; g < 8; g++) ubg[g] = 0;
+      for (int g = 0; g < 8; g++) ubt8[g] = 0;
+      unsigned long long *cb = cub + (int64_t)t * 27 * 8;
       for (int q = 0; q < 27; q++) {
         int ei, ej, ek;
         walk_off(q, 1, &ei, &ej, &ek);
@@ -736,8 +744,14 @@ k_centre_task(SodCfg c, const int32_t *__restrict__ cell_start,
           const float nb0[3] = {(float)di, (float)dj, (float)dk};
           const float nb1[3] = {(float)di + 1.f, (float)dj + 1.f,
                                 (float)dk + 1.f};
-          if (box_gap2(cb0, cb1, nb0, nb1) > cut) continue;
+          if (box_gap2(cb0, cb1, nb0, nb1) > cut) {
+            if (l < 8) cb[q * 8 + l] = 0;
+            continue;
+          }
         }
+        uint64_t ubg[8];  // this cell's part
+#pragma unroll
+        for (int g = 0; g < 8; g++) ubg[g] = 0;
         int32_t ln[3];
         const int32_t key = key_at(c.nx, pl.cs, di, dj, dk, ln);
         const int32_t a = cell_start[key], b = cell_start[key + 1];
@@ -756,12 +770,18 @@ k_centre_task(SodCfg c, const int32_t *__restrict__ cell_start,
             }
           }
         }
+#pragma unroll
+        for (int g = 0; g < 8; g++) {
+          const uint64_t v = g < ngr ? wave_sum(ubg[g]) : 0;
+          ubt8[g] += v;
+          if (l == g) cb[q * 8 + g] = v;
+        }
       }
       uint64_t ubmax = 0;
 #pragma unroll
       for (int g = 0; g < 8; g++) {
-        const uint64_t v = wave_sum(ubg[g]);
-        if (g < ngr) ubmax = v > ubmax ? v : ubmax;
+        if (l == g) cub[((int64_t)nt * 27 + t) * 8 + g] = ubt8[g];
+        if (g < ngr) ubmax = ubt8[g] > ubmax ? ubt8[g] : ubmax;
       }
       if (l == 0) {
         ubt[t] = ubmax;
@@ -771,7 +791,29 @@ k_centre_task(SodCfg c, const int32_t *__restrict__ cell_start,
     }
     uint64_t acc = 0;
     uint64_t npair = 0;  // neighbours tested per candidate
-    for (int q = 0; q < 27; q++) {
+    // bound of the cells not yet summed, for this lane's group
+    const unsigned long long *cb = cub + (int64_t)t * 27 * 8;
+    unsigned long long rem =
+        phase == 2 ? cub[((int64_t)nt * 27 + t) * 8 + (l >> 3)] : 0ull;
+    bool aborted = false;
+    for (int qi = 0; qi < 27; qi++) {
+      const int q = kCellOrder[qi];
+      if (phase == 2 && qi > 0) {
+        // exact sums so far plus the bound of the remaining cells
+        const unsigned long long ubc = have ? acc + rem : 0ull;
+        unsigned long long m = ubc;
+        for (int o = 32; o > 0; o >>= 1) {
+          const unsigned long long v = __shfl_xor(m, o, 64);
+          m = v > m ? v : m;
+        }
+        const unsigned long long best = __hip_atomic_load(
+            &hbest[tk.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (m < best) {
+          aborted = true;
+          break;
+        }
+      }
+      if (phase == 2) rem -= cb[q * 8 + (l >> 3)];
       int ei, ej, ek;
       walk_off(q, 1, &ei, &ej, &ek);
       const int di = ci + ei, dj = cj + ej, dk = ck + ek;
@@ -825,6 +867,15 @@ k_centre_task(SodCfg c, const int32_t *__restrict__ cell_start,
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       }
+    }
+    if (aborted) {
+      if (l == 0) {
+        taskres[2 * (int64_t)t] = 0;
+        taskres[2 * (int64_t)t + 1] = ~0ull;
+        atomicAdd(&cstat[3], 1ull);
+        atomicAdd(&cstat[2], (unsigned long long)(npair * (uint64_t)tk.w));
+      }
+      continue;
     }
     uint64_t key = have ? (((uint64_t)(uint32_t)tk.y << 32) | (uint32_t)pidx)
                         : ~0ull;
@@ -1602,7 +1653,7 @@ extern "C" int psod_sod_fof(psod_ctx *ctx, const psod_sod_params *p,
   const int32_t nlarge = hc[1];
   ctx->sod_stat[0] = nhalo - nlarge;
   ctx->sod_stat[1] = nlarge;
-  for (int i = 2; i < 9; i++) ctx->sod_stat[i] = 0;
+  for (int i = 2; i < 10; i++) ctx->sod_stat[i] = 0;
   if (nlarge == 0) return PSOD_OK;
   // largest halos first, so that the persistent growth workgroups do not
   // finish on a giant halo picked up last
@@ -1636,11 +1687,14 @@ extern "C" int psod_sod_fof(psod_ctx *ctx, const psod_sod_params *p,
   PSOD_HIP(hipMemcpyAsync(&nt, ntot, 4, hipMemcpyDeviceToHost, st));
   PSOD_HIP(hipStreamSynchronize(st));
   if (nt > 0) {
-    int4 *tasks = (int4 *)psod_scratch(ctx, S_TASKRES,
-                                       (size_t)nt * (sizeof(int4) + 24));
+    // tasks, their results, bounds, and bounds per (cell, group) plus the
+    // per-group totals (28 x 8 per task)
+    int4 *tasks = (int4 *)psod_scratch(
+        ctx, S_TASKRES, (size_t)nt * (sizeof(int4) + 24 + 28 * 8 * 8));
     if (!tasks) return PSOD_E_NOMEM;
     uint64_t *tres = (uint64_t *)(tasks + nt);
     unsigned long long *ubt = (unsigned long long *)(tres + 2 * (int64_t)nt);
+    unsigned long long *cub = ubt + nt;
     k_task_emit<<<grid_for(nlarge, 256), 256, 0, st>>>(
         c, cat->cell_start, plan, large, nlarge, ntask, task0, tasks);
     PSOD_LAUNCH_CHECK();
@@ -1655,7 +1709,7 @@ extern "C" int psod_sod_fof(psod_ctx *ctx, const psod_sod_params *p,
                       256, 0, st>>>(c, cat->cell_start,
                                     (const float4 *)cat->rec, plan, large,
                                     tasks, ntot, tres, phase, hbest, hpil,
-                                    ubt, cstat);
+                                    ubt, cub, cstat);
       PSOD_LAUNCH_CHECK();
     }
     k_centre_reduce<<<grid_for(nlarge, 256), 256, 0, st>>>(plan, large,
@@ -1665,6 +1719,8 @@ extern "C" int psod_sod_fof(psod_ctx *ctx, const psod_sod_params *p,
     // read back with the growth's counters (the stream is synchronised
     // before this call returns)
     PSOD_HIP(hipMemcpyAsync(ctx->sod_stat + 4, cstat, 3 * 8,
+                            hipMemcpyDeviceToHost, st));
+    PSOD_HIP(hipMemcpyAsync(ctx->sod_stat + 9, cstat + 3, 8,
                             hipMemcpyDeviceToHost, st));
     ctx->sod_stat[7] = nt;
   }
