@@ -755,11 +755,23 @@ k_centre_task(SodCfg c, const int32_t *__restrict__ cell_start,
         int32_t ln[3];
         const int32_t key = key_at(c.nx, pl.cs, di, dj, dk, ln);
         const int32_t a = cell_start[key], b = cell_start[key + 1];
-        for (int32_t s = a + l; s < b; s += 64) {
-          const float4 r = rec[s];
-          const float vx = rel1(r.x, ln[0], di, 0.0f);
-          const float vy = rel1(r.y, ln[1], dj, 0.0f);
-          const float vz = rel1(r.z, ln[2], dk, 0.0f);
+        for (int32_t s0 = a; s0 < b; s0 += 64) {
+          const int32_t s = s0 + l;
+          float vx = 0.f, vy = 0.f, vz = 0.f;
+          bool near = false;
+          if (s < b) {
+            const float4 r = rec[s];
+            vx = rel1(r.x, ln[0], di, 0.0f);
+            vy = rel1(r.y, ln[1], dj, 0.0f);
+            vz = rel1(r.z, ln[2], dk, 0.0f);
+            // within reach of the task's box (its groups lie inside it)
+            const float ex = fmaxf(0.0f, fmaxf(cb0[0] - vx, vx - cb1[0]));
+            const float ey = fmaxf(0.0f, fmaxf(cb0[1] - vy, vy - cb1[1]));
+            const float ez = fmaxf(0.0f, fmaxf(cb0[2] - vz, vz - cb1[2]));
+            near = ex * ex + ey * ey + ez * ez <= cut;
+          }
+          if (__ballot(near) == 0) continue;  // the whole chunk is beyond reach
+          if (!near) continue;
 #pragma unroll
           for (int g = 0; g < 8; g++) {
             if (g < ngr) {
