@@ -627,6 +627,10 @@ __constant__ int8_t kCellOrder[27] = {13, 4,  10, 12, 14, 16, 22, 1,  3,
                                       5,  7,  9,  11, 15, 17, 19, 21, 23,
                                       25, 0,  2,  6,  8,  18, 20, 24, 26};
 
+// halos with fewer particles in the 5^3 cells around the seed skip the
+// bound: bounding costs about as much as their exact sums
+constexpr int32_t BB_MIN = 2048;
+
 // Upper bound of the term of any pair at squared distance >= g2 (the float
 // d2 of a pair is at least g2 up to a few ulps, hence the 2e-5 shrink; +4
 // covers non-monotone rounding of the reciprocal square root)
@@ -683,6 +687,15 @@ k_centre_task(SodCfg c, const int32_t *__restrict__ cell_start,
       }
     }
     const HaloPlan pl = plan[large[tk.x]];
+    if (phase == 0 && pl.n5 < BB_MIN) {
+      // few particles around the seed: every task of the halo is evaluated
+      // exactly in phase 1 (an all-ones bound makes each one a pilot)
+      if (l == 0) {
+        ubt[t] = ~0ull;
+        atomicMax(&hpil[tk.x], ~0ull);
+      }
+      continue;
+    }
     int ci, cj, ck;
     walk_off(tk.y, 1, &ci, &cj, &ck);
     int32_t lc[3];
