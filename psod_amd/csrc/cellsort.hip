@@ -61,77 +61,179 @@ __device__ inline PartCell part_cell(const float *pos, int64_t i, int32_t nx) {
   return p;
 }
 
-// Wave-aggregated "atomicAdd(&cnt[key], 1)" returning each lane's slot:
-// lanes with the leader's key share one atomic (particles of one clump are
-// usually consecutive in the input).  After a few rounds the remaining
-// lanes fall back to one atomic each.  Every lane of the wave must call it.
-__device__ inline int32_t agg_slot(int32_t *cnt, int32_t key, bool valid,
-                                   bool want_slot) {
-  const int l = __lane_id();
-  uint64_t act = __ballot(valid);
-  int32_t slot = 0;
-  for (int it = 0; it < 4 && act; it++) {
-    const int leader = __ffsll((unsigned long long)act) - 1;
-    const int32_t kl = __shfl(key, leader, 64);
-    const bool mine = valid && ((act >> l) & 1) && key == kl;
-    const uint64_t m = __ballot(mine);
-    int32_t base = 0;
-    if (l == leader) {
-      if (want_slot) base = atomicAdd(&cnt[kl], __popcll(m));
-      else atomicAdd(&cnt[kl], __popcll(m));
-    }
-    base = __shfl(base, leader, 64);
-    if (mine) {
-      const uint64_t lower = (l == 0) ? 0ull : (~0ull >> (64 - l));
-      slot = base + __popcll(m & lower);
-    }
-    act &= ~m;
-  }
-  if ((act >> l) & 1) {
-    if (want_slot) slot = atomicAdd(&cnt[key], 1);
-    else atomicAdd(&cnt[key], 1);
-  }
-  return slot;
+// ---------------------------------------------------------------------------
+// Records into block order without global atomics: an MSD radix sort of the
+// block id (3 log2(nx/8) bits) in passes of at most 7 bits, each pass
+// splitting every bucket of the previous one into at most 128.  A pass works
+// on tiles of CS_T elements inside one bucket.  A workgroup counts its tile
+// per digit in LDS; the counts of all tiles are scanned (bucket by bucket,
+// digit-major, so that the scan gives each (digit, tile) its first slot);
+// the workgroup then places its tile in chunks of CS_C elements: a counting
+// sort of the chunk in LDS followed by writes in digit order, so that
+// consecutive lanes write consecutive slots.  The first pass reads the
+// positions and builds the records.  The order inside a block is not kept:
+// the block sort below orders it by cell.
+// ---------------------------------------------------------------------------
+constexpr int32_t CS_T = 1 << 16;  // elements per tile
+constexpr int CS_C = 2048;         // elements per LDS chunk
+constexpr int CS_U = CS_C / 256;   // per thread
+constexpr int CS_DB = 7;           // digit bits per pass
+
+// tile t of a pass: bucket b, its j-th tile, elements [k0, k1)
+__device__ inline bool cs_tile(int32_t t, const int32_t *tb,
+                               const int32_t *tmap, const int32_t *bs,
+                               int32_t nbk, int32_t *b, int32_t *j,
+                               int32_t *k0, int32_t *k1) {
+  if (t >= tb[nbk]) return false;
+  *b = tmap[t];
+  *j = t - tb[*b];
+  *k0 = bs[*b] + *j * CS_T;
+  *k1 = min(bs[*b + 1], *k0 + CS_T);
+  return true;
 }
 
-__global__ void __launch_bounds__(256)
-k_block_hist(const float *__restrict__ pos, int64_t n, int32_t nx,
-             int32_t *__restrict__ bcount) {
-  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n;
-       base += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t i = base + threadIdx.x;
-    const bool ok = i < n;
-    int32_t blk = 0;
-    if (ok) blk = part_cell(pos, i, nx).blk;
-    agg_slot(bcount, blk, ok, false);
+// element k of a pass: its record, block and label
+template <bool FROM_POS>
+__device__ inline void cs_load(int64_t k, const float *pos, const int32_t *lab,
+                               const float4 *sin, const int32_t *bin,
+                               const int32_t *lin, int32_t nx, float4 *r,
+                               int32_t *blk, int32_t *lb) {
+  if (FROM_POS) {
+    const PartCell p = part_cell(pos, k, nx);
+    r->x = p.xs - (float)(p.cx & ~7);
+    r->y = p.ys - (float)(p.cy & ~7);
+    r->z = p.zs - (float)(p.cz & ~7);
+    r->w = __int_as_float((int32_t)k);
+    *blk = p.blk;
+    *lb = lab ? lab[k] : -1;
+  } else {
+    *r = sin[k];
+    *blk = bin[k];
+    *lb = lin ? lin[k] : -1;
   }
 }
 
-// records staged in their block's region, unsorted inside it
+template <bool FROM_POS>
 __global__ void __launch_bounds__(256)
-k_block_scatter(const float *__restrict__ pos, const int32_t *__restrict__ lab,
-                int64_t n, int32_t nx, const int32_t *__restrict__ bstart,
-                int32_t *__restrict__ bfill, float4 *__restrict__ stage,
-                int32_t *__restrict__ stage_lab) {
-  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n;
-       base += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t i = base + threadIdx.x;
-    const bool ok = i < n;
-    PartCell p;
-    p.blk = 0;
-    if (ok) p = part_cell(pos, i, nx);
-    const int32_t s = agg_slot(bfill, p.blk, ok, true);
-    if (ok) {
-      const int32_t slot = bstart[p.blk] + s;
-      float4 r;
-      r.x = p.xs - (float)(p.cx & ~7);
-      r.y = p.ys - (float)(p.cy & ~7);
-      r.z = p.zs - (float)(p.cz & ~7);
-      r.w = __int_as_float((int32_t)i);
-      stage[slot] = r;
-      if (stage_lab) stage_lab[slot] = lab ? lab[i] : -1;
-    }
+k_cs_hist(const float *__restrict__ pos, const int32_t *__restrict__ bin,
+          int32_t nx, int shift, int dbits, int32_t nbk,
+          const int32_t *__restrict__ bs, const int32_t *__restrict__ ntb,
+          const int32_t *__restrict__ tb, const int32_t *__restrict__ tmap,
+          int32_t *__restrict__ H) {
+  __shared__ int32_t hist[1 << CS_DB];
+  int32_t b, j, k0, k1;
+  if (!cs_tile(blockIdx.x, tb, tmap, bs, nbk, &b, &j, &k0, &k1)) return;
+  const int32_t nd = 1 << dbits;
+  for (int d = threadIdx.x; d < nd; d += 256) hist[d] = 0;
+  __syncthreads();
+  for (int32_t k = k0 + threadIdx.x; k < k1; k += 256) {
+    const int32_t blk = FROM_POS ? part_cell(pos, k, nx).blk : bin[k];
+    atomicAdd(&hist[(blk >> shift) & (nd - 1)], 1);
   }
+  __syncthreads();
+  int32_t *reg = H + (int64_t)tb[b] * nd;  // bucket b: digit-major, tiles minor
+  for (int d = threadIdx.x; d < nd; d += 256) reg[(int64_t)d * ntb[b] + j] = hist[d];
+}
+
+template <bool FROM_POS>
+__global__ void __launch_bounds__(256)
+k_cs_scatter(const float *__restrict__ pos, const int32_t *__restrict__ lab,
+             const float4 *__restrict__ sin, const int32_t *__restrict__ bin,
+             const int32_t *__restrict__ lin, int32_t nx, int shift, int dbits,
+             int32_t nbk, const int32_t *__restrict__ bs,
+             const int32_t *__restrict__ ntb, const int32_t *__restrict__ tb,
+             const int32_t *__restrict__ tmap, const int32_t *__restrict__ O,
+             float4 *__restrict__ sout, int32_t *__restrict__ bout,
+             int32_t *__restrict__ lout) {
+  __shared__ float4 sr[CS_C];
+  __shared__ int32_t sbk[CS_C], slb[CS_C];
+  __shared__ int32_t gfill[1 << CS_DB], lcnt[1 << CS_DB], lst[1 << CS_DB];
+  int32_t b, j, k0, k1;
+  if (!cs_tile(blockIdx.x, tb, tmap, bs, nbk, &b, &j, &k0, &k1)) return;
+  const int32_t nd = 1 << dbits;
+  const int32_t *reg = O + (int64_t)tb[b] * nd;
+  for (int d = threadIdx.x; d < nd; d += 256) gfill[d] = reg[(int64_t)d * ntb[b] + j];
+  for (int32_t c0 = k0; c0 < k1; c0 += CS_C) {
+    const int32_t cn = min(CS_C, k1 - c0);
+    for (int d = threadIdx.x; d < nd; d += 256) lcnt[d] = 0;
+    __syncthreads();
+    float4 rr[CS_U];
+    int32_t bb[CS_U], ll[CS_U], rk[CS_U];
+#pragma unroll
+    for (int u = 0; u < CS_U; u++) {
+      const int32_t i = threadIdx.x + 256 * u;
+      rk[u] = 0;
+      if (i < cn) {
+        cs_load<FROM_POS>(c0 + i, pos, lab, sin, bin, lin, nx, &rr[u], &bb[u],
+                          &ll[u]);
+        rk[u] = atomicAdd(&lcnt[(bb[u] >> shift) & (nd - 1)], 1);
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {  // exclusive scan of the chunk's counts (nd <= 128)
+      const int d0 = 2 * threadIdx.x;
+      const int32_t v0 = d0 < nd ? lcnt[d0] : 0, v1 = d0 + 1 < nd ? lcnt[d0 + 1] : 0;
+      const int32_t inc = wave_incl_scan(v0 + v1);
+      if (d0 < nd) lst[d0] = inc - v0 - v1;
+      if (d0 + 1 < nd) lst[d0 + 1] = inc - v1;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < CS_U; u++) {
+      const int32_t i = threadIdx.x + 256 * u;
+      if (i < cn) {
+        const int32_t p = lst[(bb[u] >> shift) & (nd - 1)] + rk[u];
+        sr[p] = rr[u];
+        sbk[p] = bb[u];
+        slb[p] = ll[u];
+      }
+    }
+    __syncthreads();
+    for (int32_t i = threadIdx.x; i < cn; i += 256) {
+      const int32_t d = (sbk[i] >> shift) & (nd - 1);
+      const int32_t dst = gfill[d] + (i - lst[d]);
+      sout[dst] = sr[i];
+      bout[dst] = sbk[i];
+      if (lout) lout[dst] = slb[i];
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < nd; d += 256) gfill[d] += lcnt[d];
+    __syncthreads();
+  }
+}
+
+// tiles per bucket (entry nbk is not used by the scan's total)
+__global__ void k_cs_ntiles(const int32_t *__restrict__ bs, int32_t nbk,
+                            int32_t *__restrict__ ntb) {
+  const int32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < nbk) ntb[b] = (bs[b + 1] - bs[b] + CS_T - 1) / CS_T;
+  else if (b == nbk) ntb[b] = 0;
+}
+
+__global__ void k_cs_tmap(const int32_t *__restrict__ ntb,
+                          const int32_t *__restrict__ tb, int32_t nbk,
+                          int32_t *__restrict__ tmap) {
+  const int32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbk) return;
+  for (int32_t j = 0; j < ntb[b]; j++) tmap[tb[b] + j] = b;
+}
+
+// starts of the buckets of the next pass: bucket b splits into nd
+__global__ void k_cs_split(const int32_t *__restrict__ O, int dbits,
+                           int32_t nbk, const int32_t *__restrict__ bs,
+                           const int32_t *__restrict__ ntb,
+                           const int32_t *__restrict__ tb, int64_t n,
+                           int32_t *__restrict__ bs2) {
+  const int64_t B = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int32_t nd = 1 << dbits;
+  const int64_t nb2 = (int64_t)nbk * nd;
+  if (B > nb2) return;
+  if (B == nb2) {
+    bs2[B] = (int32_t)n;
+    return;
+  }
+  const int32_t b = (int32_t)(B >> dbits), d = (int32_t)(B & (nd - 1));
+  bs2[B] = ntb[b] > 0 ? O[(int64_t)tb[b] * nd + (int64_t)d * ntb[b]] : bs[b];
 }
 
 // one workgroup per block: counting sort of its records by local cell
@@ -309,30 +411,82 @@ extern "C" int psod_cell_sort(psod_ctx *ctx, int64_t npart, const float *d_pos,
   const int64_t ncell = (int64_t)nx * nx * nx;
   const int64_t nblk = ncell / BS3;
   hipStream_t st = ctx->stream;
-  // block counters (+1 for the total), their scan and the fill counters
-  int32_t *bc = (int32_t *)psod_scratch(ctx, S_COUNT,
-                                        (size_t)(3 * (nblk + 1) + 64) * 4);
-  float4 *stage = (float4 *)psod_scratch(ctx, S_KEYS2,
-                                         (size_t)(npart > 0 ? npart : 1) * 16);
-  int32_t *slab = d_lab_sorted
-                      ? (int32_t *)psod_scratch(ctx, S_VALS2,
-                                                (size_t)(npart > 0 ? npart : 1) * 4)
-                      : nullptr;
-  if (!bc || !stage || (d_lab_sorted && !slab)) return PSOD_E_NOMEM;
-  int32_t *bstart = bc + nblk + 16, *bfill = bstart + nblk + 16;
-  PSOD_TRY(psod_fill_i32(ctx, bc, nblk + 1, 0));
-  PSOD_TRY(psod_fill_i32(ctx, bfill, nblk, 0));
-  const unsigned g = grid_for(npart, 256, (int64_t)ctx->num_cus * 64);
-  if (npart > 0) {
-    k_block_hist<<<g, 256, 0, st>>>(d_pos, npart, nx, bc);
-    PSOD_LAUNCH_CHECK();
+  const int64_t n = npart;
+  int bits = 0;
+  while ((int64_t)1 << bits < nblk) bits++;
+  // at least one pass (a 0-bit digit when there is a single block)
+  const int npass = bits > 0 ? (bits + CS_DB - 1) / CS_DB : 1;
+  const int64_t ntile0 = (n + CS_T - 1) / CS_T;
+  // bucket tables (two generations), tiles, histograms
+  const int64_t maxt = ntile0 + nblk / 2 + 64;  // tiles of any pass, at most
+  const int64_t nH = maxt << CS_DB;
+  int32_t *hb = (int32_t *)psod_scratch(
+      ctx, S_CS_H, (size_t)(2 * (nblk + 64) + 2 * (nblk + 64) + maxt + 64 +
+                            2 * nH + 256) * 4);
+  float4 *sa = (float4 *)psod_scratch(ctx, S_CS_A, (size_t)(n > 0 ? n : 1) * 24);
+  float4 *sc = (float4 *)psod_scratch(ctx, S_CS_C, (size_t)(n > 0 ? n : 1) * 24);
+  if (!hb || !sa || !sc) return PSOD_E_NOMEM;
+  int32_t *bsA = hb, *bsB = bsA + nblk + 64, *ntb = bsB + nblk + 64,
+          *tb = ntb + nblk + 64, *tmap = tb + nblk + 64, *H = tmap + maxt + 64,
+          *O = H + nH;
+  // record buffers: float4 records, then block ids, then labels
+  float4 *recb[2] = {sa, sc};
+  int32_t *blkb[2] = {(int32_t *)(sa + n), (int32_t *)(sc + n)};
+  int32_t *labb[2] = {d_lab_sorted ? blkb[0] + n : nullptr,
+                      d_lab_sorted ? blkb[1] + n : nullptr};
+  int32_t *bs = bsA, *bs2 = bsB;
+  int32_t nbk = 1;
+  int cur = 0;  // buffer holding the last pass's output
+  {
+    const int32_t h0[2] = {0, (int32_t)n};
+    PSOD_HIP(hipMemcpyAsync(bs, h0, 8, hipMemcpyHostToDevice, st));
+    PSOD_HIP(hipStreamSynchronize(st));  // h0 lives on this stack frame
   }
-  PSOD_TRY(psod_scan_i32(ctx, bc, bstart, nblk + 1, nullptr));
-  if (npart > 0) {
-    k_block_scatter<<<g, 256, 0, st>>>(d_pos, d_labels, npart, nx, bstart,
-                                       bfill, stage, slab);
+  for (int p = 0; p < npass && n > 0; p++) {
+    const int hi = bits - CS_DB * p;
+    const int dbits = hi < CS_DB ? hi : CS_DB;
+    const int shift = hi - dbits;
+    const int32_t nd = 1 << dbits;
+    const int64_t mt = ntile0 + nbk;  // tiles of this pass, at most
+    k_cs_ntiles<<<grid_for(nbk + 1, 256), 256, 0, st>>>(bs, nbk, ntb);
     PSOD_LAUNCH_CHECK();
+    PSOD_TRY(psod_scan_i32(ctx, ntb, tb, nbk + 1, nullptr));
+    k_cs_tmap<<<grid_for(nbk, 256), 256, 0, st>>>(ntb, tb, nbk, tmap);
+    PSOD_LAUNCH_CHECK();
+    PSOD_TRY(psod_fill_i32(ctx, H, mt * nd, 0));
+    const int src = cur, dst = p == 0 ? 0 : 1 - cur;
+    if (p == 0) {
+      k_cs_hist<true><<<(unsigned)mt, 256, 0, st>>>(d_pos, nullptr, nx, shift,
+                                                    dbits, nbk, bs, ntb, tb,
+                                                    tmap, H);
+    } else {
+      k_cs_hist<false><<<(unsigned)mt, 256, 0, st>>>(nullptr, blkb[src], nx,
+                                                     shift, dbits, nbk, bs,
+                                                     ntb, tb, tmap, H);
+    }
+    PSOD_LAUNCH_CHECK();
+    PSOD_TRY(psod_scan_i32(ctx, H, O, mt * nd, nullptr));
+    if (p == 0) {
+      k_cs_scatter<true><<<(unsigned)mt, 256, 0, st>>>(
+          d_pos, d_labels, nullptr, nullptr, nullptr, nx, shift, dbits, nbk,
+          bs, ntb, tb, tmap, O, recb[dst], blkb[dst], labb[dst]);
+    } else {
+      k_cs_scatter<false><<<(unsigned)mt, 256, 0, st>>>(
+          nullptr, nullptr, recb[src], blkb[src], labb[src], nx, shift, dbits,
+          nbk, bs, ntb, tb, tmap, O, recb[dst], blkb[dst], labb[dst]);
+    }
+    PSOD_LAUNCH_CHECK();
+    k_cs_split<<<grid_for((int64_t)nbk * nd + 1, 256), 256, 0, st>>>(
+        O, dbits, nbk, bs, ntb, tb, n, bs2);
+    PSOD_LAUNCH_CHECK();
+    nbk *= nd;
+    int32_t *t = bs; bs = bs2; bs2 = t;
+    cur = dst;
   }
+  if (n == 0) PSOD_TRY(psod_fill_i32(ctx, bs, nblk + 1, 0));
+  const int32_t *bstart = bs;  // nblk + 1 block starts
+  const float4 *stage = recb[cur];
+  const int32_t *slab = labb[cur];
   k_block_sort<<<grid_for(nblk, 1, (int64_t)ctx->num_cus * 32), 256, 0, st>>>(
       bstart, nblk, stage, slab, d_cell_start, (float4 *)d_rec, d_perm,
       d_lab_sorted);

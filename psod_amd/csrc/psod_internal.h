@@ -59,6 +59,9 @@ enum ScratchSlot {
   S_GREEDY2,
   S_GREEDY3,
   S_TILE,      // deposit tile counts and dispatch order
+  S_CS_A,      // cell sort: records, block ids, labels (two buffers)
+  S_CS_C,
+  S_CS_H,      // cell sort: bucket tables and tile histograms
   S_NSLOT
 };
 
