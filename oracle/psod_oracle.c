@@ -434,10 +434,14 @@ int or_sod_greedy(int64_t npart, const float *pos, const int64_t *ids,
     double *rb = rad_bins + (*nrec) * p->nbins;
     int64_t *mb = mass_bins + (*nrec) * p->nbins;
     for (int j = 1; j <= p->nbins; j++) {
-      rb[j - 1] = rayon * exp(log(p->rmin_frac) +
-                              (double)(j - 1) *
-                                  (log(p->rmax_frac) - log(p->rmin_frac)) /
-                                  ((double)p->nbins - 1.0));
+      /* the last radius is rayon * rmax_frac exactly (R itself when
+         rmax_frac = 1, which main file:2105 reaches only up to rounding) */
+      rb[j - 1] = j == p->nbins
+                      ? rayon * p->rmax_frac
+                      : rayon * exp(log(p->rmin_frac) +
+                                    (double)(j - 1) *
+                                        (log(p->rmax_frac) - log(p->rmin_frac)) /
+                                        ((double)p->nbins - 1.0));
       int64_t k = (j == 1) ? 1 : mb[j - 2];
       if (j > 1) mb[j - 1] = mb[j - 2];
       else mb[0] = 0;
@@ -722,10 +726,12 @@ int or_sod_fof(int64_t npart, const float *pos, const float *vel,
       double pm = INFINITY;
       for (int j = 0; j < p->nbins; j++) {
         const double rj =
-            R0 * exp(log(p->rmin_frac) + (double)j *
-                                             (log(p->rmax_frac) -
-                                              log(p->rmin_frac)) /
-                                             ((double)p->nbins - 1.0));
+            j == p->nbins - 1
+                ? R0 * p->rmax_frac
+                : R0 * exp(log(p->rmin_frac) + (double)j *
+                                                   (log(p->rmax_frac) -
+                                                    log(p->rmin_frac)) /
+                                                   ((double)p->nbins - 1.0));
         int64_t lo = 0, hi = ng; /* first index with rr >= rj */
         while (lo < hi) {
           const int64_t mid = (lo + hi) / 2;

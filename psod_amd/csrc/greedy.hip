@@ -25,7 +25,7 @@ constexpr int64_t GCAP = 1 << 18;  // gathered particles per workgroup slice
 
 struct GreedyCfg {
   int32_t nx, ncut, nbins, nb;
-  double seuil, seuil2, mmin, lr0, lr1;
+  double seuil, seuil2, mmin, lr0, lr1, rmaxf;
   int64_t nsel;
 };
 
@@ -357,9 +357,11 @@ k_greedy_seed(GreedyCfg c, const int32_t *__restrict__ seeds,
     const double amass = (double)masse;
     // profile (main file:2103-2115)
     for (int j = threadIdx.x; j < c.nbins; j += GT) {
+      // the last radius is rayon * rmax_frac exactly
       const double rj =
-          rayon * exp(c.lr0 + (double)j * (c.lr1 - c.lr0) /
-                                  ((double)c.nbins - 1.0));
+          j == c.nbins - 1 ? rayon * c.rmaxf
+                           : rayon * exp(c.lr0 + (double)j * (c.lr1 - c.lr0) /
+                                                     ((double)c.nbins - 1.0));
       int32_t lo2 = 0, hi2 = n1;
       while (lo2 < hi2) {
         const int32_t mid = (lo2 + hi2) >> 1;
@@ -439,6 +441,7 @@ extern "C" int psod_host_spherover(psod_ctx *ctx, int64_t npart,
   c.mmin = p->mmin;
   c.lr0 = log(p->rmin_frac);
   c.lr1 = log(p->rmax_frac);
+  c.rmaxf = p->rmax_frac;
   c.nsel = p->nsel;
   hipStream_t st = ctx->stream;
   const int64_t ncellt = (int64_t)p->nx * p->nx * p->nx;

@@ -36,8 +36,11 @@ __device__ inline double rad_for(int64_t k, double mpart, double delta) {
   return cbrt(((double)k * mpart) / (delta * (2.0 / 3.0 * TWOPI_REF)));
 }
 
-// j-th profile radius (main file:2105), r_j = R exp(lr0 + j (lr1-lr0)/(n-1))
+// j-th profile radius (main file:2105), r_j = R exp(lr0 + j (lr1-lr0)/(n-1));
+// the last one is R * rmax_frac exactly (R itself for the reference's
+// rmax_frac = 1, which the exponential gives only up to rounding)
 __device__ inline double prof_radius(const SodCfg &c, double R, int j) {
+  if (j == c.nbins - 1) return R * c.rmax_frac;
   return R * exp(c.lr0 + (double)j * (c.lr1 - c.lr0) / ((double)c.nbins - 1.0));
 }
 

@@ -109,6 +109,7 @@ def shells(nx=16, delta=200.0, kstar=40, kmax=120, knot=0.05, seed=3,
     assert od[exp_kstar - 1] <= delta and np.all(od[9:exp_kstar - 1] > delta)
     prof_r = rs[exp_kstar - 2] * np.exp(
         np.log(0.01) + np.arange(32) * (np.log(2.0) - np.log(0.01)) / 31.0)
+    prof_r[-1] = rs[exp_kstar - 2] * 2.0  # the last radius is R * rmax exactly
     prof = np.searchsorted(rs, prof_r, side="left")
     return dict(pos=pos, labels=lab, nx=np.int32(nx), delta=np.float64(delta),
                 centre=np.int64(0), count=np.int64(exp_kstar - 1),

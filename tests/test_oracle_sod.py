@@ -72,6 +72,10 @@ def test_greedy_designed_crossing(orc):
     k = int(np.where(h["centre"] == z["centre"])[0][0])
     assert h["masse"][k] == z["count"] and h["rayon"][k] == z["radius"]
     assert h["masse2"][k] == 9 and h["accepted"][k] == 1
+    # the last profile radius is R itself (rmax = 1), so its count is the
+    # particles strictly inside R: masse - 1 for distinct radii
+    assert r["rad_bins"][k][-1] == h["rayon"][k]
+    assert r["mass_bins"][k][-1] == h["masse"][k] - 1
     # the sphere's particles are excluded afterwards, accepted or not
     assert r["alive"][:int(z["count"])].sum() == 0
 
@@ -182,6 +186,7 @@ def test_profile_monotone_and_counts(orc):
         R = r["radius"][h, 0]
         rj = R * np.exp(np.log(0.01) + np.arange(32) * (np.log(2.0) -
                                                          np.log(0.01)) / 31.0)
+        rj[-1] = 2.0 * R  # the last radius is R * rmax_frac exactly
         np.testing.assert_array_equal(prof, (rr[None, :] < rj[:, None]).sum(1))
         inside = np.argsort(rr, kind="stable")[:r["count"][h, 0]]
         np.testing.assert_allclose(r["com_vel"][h], vel[inside].astype(
