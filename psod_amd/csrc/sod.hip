@@ -1036,24 +1036,81 @@ __device__ inline int32_t block_last(int32_t lo, int32_t hi, P &&pred,
   return -1;
 }
 
+
+// The cube of cells at offsets [-W, W]^3 is walked as row segments: for
+// fixed (dj, dk) the x range splits at 8^3-block boundaries, and the cells of
+// one block row are consecutive keys, so a segment's particles are one
+// contiguous run of records.  A particle's x cell is floor(u) of its record.
+struct CubeSegs {
+  int32_t X0, X1, bx0, nsegx, side, nb;
+  int64_t nseg;
+};
+
+__device__ inline int32_t floor8(int32_t x) { return x >= 0 ? x >> 3 : -((7 - x) >> 3); }
+
+__device__ inline CubeSegs cube_segs(const SodCfg &c, const CellRef &cr, int W) {
+  CubeSegs g;
+  g.X0 = cr.c[0] - W;
+  g.X1 = cr.c[0] + W;
+  g.bx0 = floor8(g.X0);
+  g.nsegx = floor8(g.X1) - g.bx0 + 1;
+  g.side = 2 * W + 1;
+  g.nb = c.nx >> 3;
+  g.nseg = (int64_t)g.side * g.side * g.nsegx;
+  return g;
+}
+
+// segment q: its block (unwrapped), row offsets, wrapped row cells, and its
+// first cell key and cell count along x
+__device__ inline void cube_seg(const SodCfg &c, const CellRef &cr,
+                                const CubeSegs &g, int W, int32_t q,
+                                int32_t *bx, int *dj, int *dk, int32_t *gy,
+                                int32_t *gz, int32_t *ka, int32_t *ncx) {
+  const int32_t row = q / g.nsegx;
+  *bx = g.bx0 + (q - row * g.nsegx);
+  *dj = row / g.side - W;
+  *dk = row % g.side - W;
+  *gy = wrapi(cr.c[1] + *dj, c.nx);
+  *gz = wrapi(cr.c[2] + *dk, c.nx);
+  const int32_t xa = max(g.X0, 8 * *bx), xb = min(g.X1, 8 * *bx + 7);
+  const int32_t gbx = wrapi(*bx, g.nb);
+  *ka = cell_key(c.nx, 8 * gbx + (xa - 8 * *bx), *gy, *gz);
+  *ncx = xb - xa + 1;
+}
+
+// particles in the cells at offsets [-W, W]^3 (cell counts only)
+__device__ inline int64_t cube_count(const SodCfg &c, GrowShared &G,
+                                     const int32_t *cell_start,
+                                     const CellRef &cr, int W) {
+  const CubeSegs g = cube_segs(c, cr, W);
+  int64_t n = 0;
+  for (int64_t q = threadIdx.x; q < g.nseg; q += GL_T) {
+    int32_t bx, gy, gz, ka, ncx;
+    int dj, dk;
+    cube_seg(c, cr, g, W, (int32_t)q, &bx, &dj, &dk, &gy, &gz, &ka, &ncx);
+    n += cell_start[ka + ncx] - cell_start[ka];
+  }
+  return block_sum_i64(n, G);
+}
+
+constexpr int CW_U = 4;
+
 // Calls f(d2, idx, rec, di, dj, dk, lc) for every particle of the cells at
 // offsets [-W, W]^3 from the reference cell.
 template <typename F>
 __device__ inline void cube_walk(const SodCfg &c, GrowShared &G,
                                  const int32_t *cell_start, const float4 *rec,
                                  const CellRef &cr, int W, F &&f) {
-  const int side = 2 * W + 1;
-  const int32_t ncell = side * side * side;
-  for (int32_t base = 0; base < ncell; base += GL_T) {
-    const int32_t q = base + threadIdx.x;
+  const CubeSegs g = cube_segs(c, cr, W);
+  for (int64_t base = 0; base < g.nseg; base += GL_T) {
+    const int64_t q = base + threadIdx.x;
     int32_t cnt = 0, st = 0;
-    if (q < ncell) {
-      int di, dj, dk;
-      walk_off(q, W, &di, &dj, &dk);
-      int32_t lc[3];
-      const int32_t key = key_at(c.nx, cr.c, di, dj, dk, lc);
-      st = cell_start[key];
-      cnt = cell_start[key + 1] - st;
+    if (q < g.nseg) {
+      int32_t bx, gy, gz, ka, ncx;
+      int dj, dk;
+      cube_seg(c, cr, g, W, (int32_t)q, &bx, &dj, &dk, &gy, &gz, &ka, &ncx);
+      st = cell_start[ka];
+      cnt = cell_start[ka + ncx] - st;
     }
     // block exclusive scan of cnt
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -1070,19 +1127,38 @@ __device__ inline void cube_walk(const SodCfg &c, GrowShared &G,
     G.st[threadIdx.x] = st;
     if (threadIdx.x == 0) G.pre[GL_T] = all;
     __syncthreads();
-    const int nq = min(GL_T, ncell - base);
-    for (int32_t j = threadIdx.x; j < all; j += GL_T) {
-      const int qq = bsearch_last_le(G.pre, nq, j);
-      const float4 r = rec[G.st[qq] + (j - G.pre[qq])];
-      int di, dj, dk;
-      walk_off(base + qq, W, &di, &dj, &dk);
-      int32_t lc[3];
-      key_at(c.nx, cr.c, di, dj, dk, lc);
-      const float dx = rel1(r.x, lc[0], di, cr.f[0]);
-      const float dy = rel1(r.y, lc[1], dj, cr.f[1]);
-      const float dz = rel1(r.z, lc[2], dk, cr.f[2]);
-      const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-      f(d2, __float_as_int(r.w), r, di, dj, dk, lc);
+    const int nq = (int)min((int64_t)GL_T, g.nseg - base);
+    // CW_U particles per thread per step: their searches and loads are
+    // independent, so their latencies overlap
+    for (int32_t j0 = threadIdx.x; j0 < all; j0 += CW_U * GL_T) {
+      int qq[CW_U];
+      float4 rv[CW_U];
+#pragma unroll
+      for (int u = 0; u < CW_U; u++) {
+        const int32_t j = j0 + u * GL_T;
+        qq[u] = 0;
+        if (j < all) {
+          qq[u] = bsearch_last_le(G.pre, nq, j);
+          rv[u] = rec[G.st[qq[u]] + (j - G.pre[qq[u]])];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < CW_U; u++) {
+        if (j0 + u * GL_T >= all) continue;
+        const float4 r = rv[u];
+        int32_t bx, gy, gz, ka, ncx;
+        int dj, dk;
+        cube_seg(c, cr, g, W, (int32_t)(base + qq[u]), &bx, &dj, &dk, &gy,
+                 &gz, &ka, &ncx);
+        const int32_t lx = (int32_t)r.x;  // u in [0, 8): the cell in the block
+        const int di = 8 * bx + lx - cr.c[0];
+        int32_t lc[3] = {lx, gy & 7, gz & 7};
+        const float dx = rel1(r.x, lc[0], di, cr.f[0]);
+        const float dy = rel1(r.y, lc[1], dj, cr.f[1]);
+        const float dz = rel1(r.z, lc[2], dk, cr.f[2]);
+        const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        f(d2, __float_as_int(r.w), r, di, dj, dk, lc);
+      }
     }
     __syncthreads();
   }
@@ -1108,22 +1184,6 @@ __device__ inline double rad3_for(int64_t k, double mpart, double delta) {
   return ((double)k * mpart) / (delta * (2.0 / 3.0 * TWOPI_REF));
 }
 
-// particles in the cells at offsets [-W, W]^3 (cell counts only)
-__device__ inline int64_t cube_count(const SodCfg &c, GrowShared &G,
-                                     const int32_t *cell_start,
-                                     const CellRef &cr, int W) {
-  const int side = 2 * W + 1;
-  const int32_t ncell = side * side * side;
-  int64_t n = 0;
-  for (int32_t q = threadIdx.x; q < ncell; q += GL_T) {
-    int di, dj, dk;
-    walk_off(q, W, &di, &dj, &dk);
-    int32_t lc[3];
-    const int32_t key = key_at(c.nx, cr.c, di, dj, dk, lc);
-    n += cell_start[key + 1] - cell_start[key];
-  }
-  return block_sum_i64(n, G);
-}
 
 // Pool slots reserved for n collected keys: n itself when the keys are
 // sorted in LDS, the next power of two otherwise (padding for the bitonic
