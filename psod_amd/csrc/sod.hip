@@ -935,7 +935,7 @@ k_centre_reduce(HaloPlan *__restrict__ plan, const int32_t *__restrict__ large,
 // ---------------------------------------------------------------------------
 // large halos: growth with radial histograms, one workgroup per halo
 // ---------------------------------------------------------------------------
-constexpr int GL_T = 256;
+constexpr int GL_T = 512;
 constexpr int SORT_CAP = 4096;
 // segments of possible crossing bins per threshold; ranges of collected
 // bins (the last slot holds the second threshold's band)
@@ -1809,7 +1809,8 @@ extern "C" int psod_sod_fof(psod_ctx *ctx, const psod_sod_params *p,
                             hipMemcpyDeviceToHost, st));
     ctx->sod_stat[7] = nt;
   }
-  const unsigned ngrow = (unsigned)min(nlarge, ctx->num_cus * 2);
+  // one 512-thread workgroup per CU (the kernel's registers allow 8 waves)
+  const unsigned ngrow = (unsigned)min(nlarge, ctx->num_cus);
   int32_t *hist = (int32_t *)psod_scratch(ctx, S_BINS,
                                           (size_t)ngrow * HIST_SLICE * 4);
   // keys per workgroup slice; PSOD_SLICE_KEYS lowers it (tests of the
@@ -1868,7 +1869,7 @@ extern "C" int psod_sod_fof(psod_ctx *ctx, const psod_sod_params *p,
     }
     const int64_t nfit = budget / (cap2 * 8);
     const unsigned ngrow2 = (unsigned)std::min<int64_t>(
-        std::min<int64_t>(nfail, nfit), (int64_t)ctx->num_cus * 2);
+        std::min<int64_t>(nfail, nfit), (int64_t)ngrow);  // histogram slices
     uint64_t *pool2 = (uint64_t *)psod_scratch(ctx, S_COLLECT,
                                                (size_t)cap2 * 8 * ngrow2);
     if (!pool2) return PSOD_E_NOMEM;
