@@ -196,6 +196,18 @@ int psod_sod_fof_plain(psod_ctx *ctx, const psod_sod_params *p,
 int psod_label_counts(psod_ctx *ctx, int64_t npart, const int32_t *d_labels,
                       int64_t nhalo, int32_t *d_nmem);
 
+/* The whole FoF-seeded pass in one call, the chain of the reference's zone
+ * loop (main file:659-744) for labelled halos: psod_cell_sort ->
+ * psod_cic_deposit (shift 0) -> psod_cic_interp with seed keys ->
+ * psod_seed_index -> psod_label_counts -> psod_sod_fof.  Device pointers;
+ * d_vel and d_ids may be NULL; d_grid [ng^3] receives rho/rho_mean when not
+ * NULL (else the context keeps it).  Intermediates are context-owned and
+ * reused by the next call.  Same status codes as psod_sod_fof. */
+int psod_pass(psod_ctx *ctx, const psod_sod_params *p, int64_t npart,
+              int32_t nx, int32_t ng, const float *d_pos, const float *d_vel,
+              const int64_t *d_ids, const int32_t *d_labels, int64_t nhalo,
+              float *d_grid, const psod_halo_out *out);
+
 /* ---- synthetic catalogue (no reference counterpart: benchmark input) ------
  * Seeded NFW clumps on a uniform background; see or_gen_* in the oracle for
  * the same specification on the CPU. */

@@ -226,6 +226,19 @@ class Context:
                      _ptr(nmem), C.byref(ho)))
         return o
 
+    def run_pass(self, p, pos, labels, nhalo, ng=None, vel=None, ids=None,
+                 grid=None, out=None):
+        """psod_pass: cell sort, CIC, seeds, member counts and the batched
+        SOD of every label in one call (device tensors)."""
+        nx = p.nx
+        ng = nx if ng is None else ng
+        o = out if out is not None else self.alloc_halo_out(nhalo, p.nbins)
+        ho = _lib.HaloOut(**{k: _ptr(o[k]).value for k in o})
+        _lib.check(_L.psod_pass(self.h, C.byref(p), pos.shape[0], nx, ng,
+                                _ptr(pos), _ptr(vel), _ptr(ids), _ptr(labels),
+                                nhalo, _ptr(grid), C.byref(ho)))
+        return o
+
     def sod_stats(self):
         """Execution-path counts of the last sod_fof call: halos grown by
         one wave, by one workgroup, re-run with larger collection slices,

@@ -17,7 +17,7 @@ F_VELOCITY = 1
 
 EXPORTS = [
     "psod_create", "psod_destroy", "psod_last_error", "psod_synchronize",
-    "psod_stream", "psod_scratch_bytes", "psod_sod_stats", "psod_cell_sort", "psod_cic_deposit",
+    "psod_stream", "psod_scratch_bytes", "psod_sod_stats", "psod_pass", "psod_cell_sort", "psod_cic_deposit",
     "psod_cic_interp", "psod_seed_index", "psod_density_order",
     "psod_sod_fof", "psod_sod_fof_plain", "psod_label_counts",
     "psod_gen_clumps", "psod_generate", "psod_host_cic", "psod_host_cicm1",
@@ -114,6 +114,8 @@ def load():
         "psod_sod_fof_plain": (C.c_int, [v, P(SodParams), P(Catalogue), i64,
                                          v, v, P(HaloOut)]),
         "psod_label_counts": (C.c_int, [v, i64, v, i64, v]),
+        "psod_pass": (C.c_int, [v, P(SodParams), i64, i32, i32, v, v, v, v,
+                                i64, v, P(HaloOut)]),
         "psod_gen_clumps": (i64, [P(GenParams), v, v, v, v, v, v]),
         "psod_generate": (C.c_int, [v, P(GenParams), v, v, v, v, v, v, v, v,
                                     v]),

@@ -62,6 +62,11 @@ enum ScratchSlot {
   S_CS_A,      // cell sort: records, block ids, labels (two buffers)
   S_CS_C,
   S_CS_H,      // cell sort: bucket tables and tile histograms
+  S_PASS_CS,   // psod_pass: cell offsets, records, labels, halo arrays, grid
+  S_PASS_REC,
+  S_PASS_LAB,
+  S_PASS_HALO,
+  S_PASS_GRID,
   S_NSLOT
 };
 

@@ -231,6 +231,29 @@ def test_sod_retry_with_larger_slices(ctx, c64, monkeypatch):
         np.testing.assert_array_equal(a[k], b[k], err_msg=k)
 
 
+def test_whole_pass_matches_the_chain(ctx, c64):
+    """psod_pass gives exactly the outputs of the step-by-step chain."""
+    import psod_amd as P
+    import torch
+    nx, N, nh = c64["nx"], c64["N"], c64["nh"]
+    p = P.sod_params(nx, N, deltas=(200.0, 500.0), delta2=800.0,
+                     velocity=True)
+    cat = ctx.cell_sort(c64["pos"], nx, labels=c64["lab"])
+    grid = ctx.cic_deposit(cat, nx)
+    seeds = ctx.seeds(cat, grid, nh)
+    nmem = ctx.label_counts(c64["lab"], nh)
+    a = ctx.sod_fof(p, cat, c64["pos"], seeds, nmem, vel=c64["vel"],
+                    ids=c64["ids"])
+    a = {k: host(v) for k, v in a.items()}
+    g2 = torch.empty_like(grid)
+    b = ctx.run_pass(p, c64["pos"], c64["lab"], nh, vel=c64["vel"],
+                     ids=c64["ids"], grid=g2)
+    b = {k: host(v) for k, v in b.items()}
+    np.testing.assert_array_equal(host(grid), host(g2))
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+
+
 def test_sod_fof_c1(ctx, orc):
     c1 = catalogue(ctx, 128, 2000, 20000, 1)
     o, r = sod_compare(ctx, orc, c1)
