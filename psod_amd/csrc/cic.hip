@@ -225,35 +225,50 @@ k_cic_deposit(const int32_t *__restrict__ cell_start,
         atomicAdd(&acc[(az * NN + ay) * NN + ax], ra[c]);
     }
   };
-  // lane l of wave w takes list positions base + w*64*K + l*K + j, j < K
+  // lane l of wave w takes list positions base + w*64*K + l*K + j, j < K.
+  // All K records of a lane are requested before the first is used, so that
+  // a wave keeps K loads per lane in flight.
   const int wv = threadIdx.x >> 6;
   for (int32_t base = 0; base < ntot; base += 256 * DEP_K) {
+    float4 rv[DEP_K];
+    int32_t ob[DEP_K];  // block offset of the record, packed; 0 = none
+#pragma unroll
     for (int j = 0; j < DEP_K; j++) {
       const int32_t t = base + wv * 64 * DEP_K + l * DEP_K + j;
-      if (t >= ntot) break;
-      int32_t k = 0, bxo = 0, byo = 0, bzo = 0;
-      if (t < nown) {
-        int r = 0;
-        while (r + 1 < TB * TB && opre[r + 1] <= t) r++;
-        k = ostart[r] + (t - opre[r]);
-        byo = r % TB;
-        bzo = r / TB;
-        for (int q = 1; q < TB; q++)
-          if (t >= obnd[r][q]) bxo = q;
-      } else {
-        const int32_t th = t - nown;
-        int lo = 0, hi = NS;  // last s with hpre[s] <= th
-        while (hi - lo > 1) {
-          const int mid = (lo + hi) >> 1;
-          if (hpre[mid] <= th) lo = mid; else hi = mid;
+      ob[j] = 0;
+      if (t < ntot) {
+        int32_t k = 0, bxo = 0, byo = 0, bzo = 0;
+        if (t < nown) {
+          int r = 0;
+          while (r + 1 < TB * TB && opre[r + 1] <= t) r++;
+          k = ostart[r] + (t - opre[r]);
+          byo = r % TB;
+          bzo = r / TB;
+          for (int q = 1; q < TB; q++)
+            if (t >= obnd[r][q]) bxo = q;
+        } else {
+          const int32_t th = t - nown;
+          int lo = 0, hi = NS;  // last s with hpre[s] <= th
+          while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (hpre[mid] <= th) lo = mid; else hi = mid;
+          }
+          k = hstart[lo] + (th - hpre[lo]);
+          bxo = hoff[lo][0];
+          byo = hoff[lo][1];
+          bzo = hoff[lo][2];
         }
-        k = hstart[lo] + (th - hpre[lo]);
-        bxo = hoff[lo][0];
-        byo = hoff[lo][1];
-        bzo = hoff[lo][2];
+        ob[j] = ((bzo + 2) << 8) | ((byo + 2) << 4) | (bxo + 2);
+        rv[j] = rec[k];
       }
+    }
+#pragma unroll
+    for (int j = 0; j < DEP_K; j++) {
+      if (ob[j] == 0) break;
+      const int32_t bxo = (ob[j] & 15) - 2, byo = ((ob[j] >> 4) & 15) - 2,
+                    bzo = (ob[j] >> 8) - 2;
       // lower node (tile-relative) and exact offsets
-      const float4 r = rec[k];
+      const float4 r = rv[j];
       const float qx = (float)R * r.x, qy = (float)R * r.y, qz = (float)R * r.z;
       const float fx0 = floorf(qx), fy0 = floorf(qy), fz0 = floorf(qz);
       const int32_t ix = 8 * R * bxo + (int32_t)fx0;

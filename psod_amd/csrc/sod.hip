@@ -601,14 +601,24 @@ k_task_emit(SodCfg c, const int32_t *__restrict__ cell_start,
   }
 }
 
-__device__ inline float wave_minf(float v) {
-  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
-  return v;
+// Minimum / maximum over the 64 lanes of a wave, returned wave-uniform.  All
+// lanes must be active.  Row shifts by 1, 2, 4, 8 leave each 16-lane row's
+// result in its lane 15; broadcasts of lanes 15 and 31 fold the rows into
+// lane 63 (register moves between lanes, no LDS round trips).
+template <bool MAX>
+__device__ inline float wave_extf(float v) {
+  const int id = __float_as_int(MAX ? -INFINITY : INFINITY);
+  auto op = [](float a, float b) { return MAX ? fmaxf(a, b) : fminf(a, b); };
+  v = op(v, __int_as_float(__builtin_amdgcn_update_dpp(id, __float_as_int(v), 0x111, 0xf, 0xf, false)));
+  v = op(v, __int_as_float(__builtin_amdgcn_update_dpp(id, __float_as_int(v), 0x112, 0xf, 0xf, false)));
+  v = op(v, __int_as_float(__builtin_amdgcn_update_dpp(id, __float_as_int(v), 0x114, 0xf, 0xf, false)));
+  v = op(v, __int_as_float(__builtin_amdgcn_update_dpp(id, __float_as_int(v), 0x118, 0xf, 0xf, false)));
+  v = op(v, __int_as_float(__builtin_amdgcn_update_dpp(id, __float_as_int(v), 0x142, 0xa, 0xf, false)));
+  v = op(v, __int_as_float(__builtin_amdgcn_update_dpp(id, __float_as_int(v), 0x143, 0xc, 0xf, false)));
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
-__device__ inline float wave_maxf(float v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-  return v;
-}
+__device__ inline float wave_minf(float v) { return wave_extf<false>(v); }
+__device__ inline float wave_maxf(float v) { return wave_extf<true>(v); }
 
 // squared distance between two boxes (0 when they overlap)
 __device__ inline float box_gap2(const float *a0, const float *a1,
@@ -631,12 +641,220 @@ __constant__ int8_t kCellOrder[27] = {13, 4,  10, 12, 14, 16, 22, 1,  3,
 // bound: bounding costs about as much as their exact sums
 constexpr int32_t BB_MIN = 2048;
 
-// Upper bound of the term of any pair at squared distance >= g2 (the float
-// d2 of a pair is at least g2 up to a few ulps, hence the 2e-5 shrink; +4
-// covers non-monotone rounding of the reciprocal square root)
-__device__ inline uint32_t sterm_bound(float g2, const SodCfg &c) {
-  const float g = g2 * (1.0f - 2e-5f);
-  return g <= c.rs2f ? sterm_fix(g, c.eps2f) + 4u : 0u;
+// Upper bound of the term of any pair at squared distance >= g2 within the
+// cutoff.  The float d2 of a pair is at least g2 up to a few ulps, hence the
+// 2e-5 shrink; rounding with +8.5 instead of +0.5 covers non-monotone
+// rounding of the reciprocal square root.
+__device__ inline uint32_t sterm_bound(float g2, float eps2) {
+  return (uint32_t)(__builtin_amdgcn_rsqf(fmaf(g2, 1.0f - 2e-5f, eps2)) * SFIX +
+                    8.5f);
+}
+
+// Sums over the wave of u[g] for 8 groups, transposed: lane l ends with the
+// sum of group l >> 3.  Each exchange step hands half of the values still
+// held to the partner lane (10 64-bit exchanges instead of 8 x 6).
+__device__ inline uint64_t group_reduce8(const uint64_t (&u)[8], int l) {
+  const bool h5 = (l & 32) != 0, h4 = (l & 16) != 0, h3 = (l & 8) != 0;
+  uint64_t a[4], b[2];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const uint64_t keep = h5 ? u[i + 4] : u[i], send = h5 ? u[i] : u[i + 4];
+    a[i] = keep + __shfl_xor(send, 32, 64);
+  }
+#pragma unroll
+  for (int i = 0; i < 2; i++) {
+    const uint64_t keep = h4 ? a[i + 2] : a[i], send = h4 ? a[i] : a[i + 2];
+    b[i] = keep + __shfl_xor(send, 16, 64);
+  }
+  uint64_t v = (h3 ? b[1] : b[0]) + __shfl_xor(h3 ? b[0] : b[1], 8, 64);
+  v += __shfl_xor(v, 4, 64);
+  v += __shfl_xor(v, 2, 64);
+  v += __shfl_xor(v, 1, 64);
+  return v;
+}
+
+// Phase 0 of the centre search, one wave per task: each candidate's S is at
+// most the sum, over the neighbours of the 27 cells, of the term at the
+// neighbour's distance to the box of the candidate's group of 8.  Stores the
+// bound per (cell, group) in cub[t*27*8 + q*8 + g], the per-group totals in
+// cub[(nt*27 + t)*8 + g], the task's bound in ubt[t] and each halo's largest
+// task bound in hpil.
+__global__ void __launch_bounds__(256)
+k_centre_bound(SodCfg c, const int32_t *__restrict__ cell_start,
+               const float4 *__restrict__ rec, const HaloPlan *__restrict__ plan,
+               const int32_t *__restrict__ large, const int4 *__restrict__ tasks,
+               const int32_t *__restrict__ ntask_total,
+               unsigned long long *__restrict__ hpil,
+               unsigned long long *__restrict__ ubt,
+               unsigned long long *__restrict__ cub) {
+  // the wave index is uniform: task fields and everything derived from
+  // them go to scalar registers
+  const int l = threadIdx.x & 63,
+            w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int32_t nt = *ntask_total;
+  const float cut = c.rs2f * 1.0001f + 1e-6f;  // conservative chunk culling
+  for (int32_t t = blockIdx.x * 4 + w; t < nt; t += gridDim.x * 4) {
+    const int4 tk = tasks[t];
+    const HaloPlan pl = plan[large[tk.x]];
+    if (pl.n5 < BB_MIN) {
+      // few particles around the seed: every task of the halo is evaluated
+      // exactly in phase 1 (an all-ones bound makes each one a pilot)
+      if (l == 0) {
+        ubt[t] = ~0ull;
+        atomicMax(&hpil[tk.x], ~0ull);
+      }
+      continue;
+    }
+    int ci, cj, ck;
+    walk_off(tk.y, 1, &ci, &cj, &ck);
+    int32_t lc[3];
+    key_at(c.nx, pl.cs, ci, cj, ck, lc);
+    float px = 0.f, py = 0.f, pz = 0.f;
+    const bool have = l < tk.w;
+    if (have) {
+      const float4 r = rec[tk.z + l];
+      px = rel1(r.x, lc[0], ci, 0.0f);
+      py = rel1(r.y, lc[1], cj, 0.0f);
+      pz = rel1(r.z, lc[2], ck, 0.0f);
+    }
+    float cb0[3], cb1[3];
+    cb0[0] = wave_minf(have ? px : INFINITY);
+    cb0[1] = wave_minf(have ? py : INFINITY);
+    cb0[2] = wave_minf(have ? pz : INFINITY);
+    cb1[0] = wave_maxf(have ? px : -INFINITY);
+    cb1[1] = wave_maxf(have ? py : -INFINITY);
+    cb1[2] = wave_maxf(have ? pz : -INFINITY);
+    // boxes of the 8 groups of 8 candidates (uniform)
+    float g0x = have ? px : INFINITY, g0y = have ? py : INFINITY,
+          g0z = have ? pz : INFINITY;
+    float g1x = have ? px : -INFINITY, g1y = have ? py : -INFINITY,
+          g1z = have ? pz : -INFINITY;
+#pragma unroll
+    for (int o = 1; o < 8; o <<= 1) {
+      g0x = fminf(g0x, __shfl_xor(g0x, o, 64));
+      g0y = fminf(g0y, __shfl_xor(g0y, o, 64));
+      g0z = fminf(g0z, __shfl_xor(g0z, o, 64));
+      g1x = fmaxf(g1x, __shfl_xor(g1x, o, 64));
+      g1y = fmaxf(g1y, __shfl_xor(g1y, o, 64));
+      g1z = fmaxf(g1z, __shfl_xor(g1z, o, 64));
+    }
+    float B[8][6];
+#pragma unroll
+    for (int g = 0; g < 8; g++) {
+      B[g][0] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g0x), 8 * g));
+      B[g][1] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g0y), 8 * g));
+      B[g][2] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g0z), 8 * g));
+      B[g][3] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g1x), 8 * g));
+      B[g][4] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g1y), 8 * g));
+      B[g][5] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g1z), 8 * g));
+    }
+    const int ngr = (tk.w + 7) >> 3;  // groups holding candidates
+    // record ranges of the 27 cells, one per lane, all loaded at once; a
+    // cell beyond reach of the candidates' box gets an empty range
+    int32_t qa = 0, qb = 0;
+    if (l < 27) {
+      int ei, ej, ek;
+      walk_off(l, 1, &ei, &ej, &ek);
+      const int di = ci + ei, dj = cj + ej, dk = ck + ek;
+      const float nb0[3] = {(float)di, (float)dj, (float)dk};
+      const float nb1[3] = {(float)di + 1.f, (float)dj + 1.f, (float)dk + 1.f};
+      if (box_gap2(cb0, cb1, nb0, nb1) <= cut) {
+        int32_t ln[3];
+        const int32_t key = key_at(c.nx, pl.cs, di, dj, dk, ln);
+        qa = cell_start[key];
+        qb = cell_start[key + 1];
+      }
+    }
+    unsigned long long *cb = cub + (int64_t)t * 27 * 8;
+    uint64_t rest = __ballot(l < 27 && qa < qb);  // cells with records
+    for (int i = l; i < 27 * 8; i += 64)
+      if (!((rest >> (i >> 3)) & 1)) cb[i] = 0;
+    uint64_t tot = 0;    // bound of group l >> 3 over the cells done
+    uint64_t cellv = 0;  // the same for the current cell, flushed part
+    // this lane's neighbours of the current cell, per group: at most 255
+    // terms of at most 2^24 + 8 each before a flush
+    uint32_t ubg[8];
+#pragma unroll
+    for (int g = 0; g < 8; g++) ubg[g] = 0;
+    int nch = 0;  // chunks summed into ubg
+    // chunks of 64 records of the cells in turn; the next chunk is requested
+    // before the current one is used
+    int q = rest ? __ffsll((unsigned long long)rest) - 1 : 27;
+    int32_t s = q < 27 ? __builtin_amdgcn_readlane(qa, q) : 0;
+    float4 rn = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (q < 27 && s + l < __builtin_amdgcn_readlane(qb, q)) rn = rec[s + l];
+    while (q < 27) {
+      const int32_t b = __builtin_amdgcn_readlane(qb, q);
+      int ei, ej, ek;
+      walk_off(q, 1, &ei, &ej, &ek);
+      const int di = ci + ei, dj = cj + ej, dk = ck + ek;
+      const float4 r = rn;
+      const bool valid = s + l < b;
+      int qn = q;
+      int32_t sn = s + 64;
+      if (sn >= b) {
+        rest &= ~(1ull << q);
+        qn = rest ? __ffsll((unsigned long long)rest) - 1 : 27;
+        sn = qn < 27 ? __builtin_amdgcn_readlane(qa, qn) : 0;
+      }
+      if (qn < 27 && sn + l < __builtin_amdgcn_readlane(qb, qn)) rn = rec[sn + l];
+      // block-local coordinates of the walked cell (8 divides nx)
+      const int32_t lx = (pl.cs[0] + di) & 7, ly = (pl.cs[1] + dj) & 7,
+                    lz = (pl.cs[2] + dk) & 7;
+      float vx = 0.f, vy = 0.f, vz = 0.f;
+      bool near = false;
+      if (valid) {
+        vx = rel1(r.x, lx, di, 0.0f);
+        vy = rel1(r.y, ly, dj, 0.0f);
+        vz = rel1(r.z, lz, dk, 0.0f);
+        // within reach of the task's box (its groups lie inside it)
+        const float ex = fmaxf(0.0f, fmaxf(cb0[0] - vx, vx - cb1[0]));
+        const float ey = fmaxf(0.0f, fmaxf(cb0[1] - vy, vy - cb1[1]));
+        const float ez = fmaxf(0.0f, fmaxf(cb0[2] - vz, vz - cb1[2]));
+        near = ex * ex + ey * ey + ez * ez <= cut;
+      }
+      if (near) {
+#pragma unroll
+        for (int g = 0; g < 8; g++) {
+          if (g < ngr) {
+            // distance to the box per axis: the coordinate minus its clamp
+            const float gx = vx - __builtin_amdgcn_fmed3f(vx, B[g][0], B[g][3]);
+            const float gy = vy - __builtin_amdgcn_fmed3f(vy, B[g][1], B[g][4]);
+            const float gz = vz - __builtin_amdgcn_fmed3f(vz, B[g][2], B[g][5]);
+            const float g2 = fmaf(gx, gx, fmaf(gy, gy, gz * gz));
+            ubg[g] += g2 <= cut ? sterm_bound(g2, c.eps2f) : 0u;
+          }
+        }
+      }
+      if (qn != q || ++nch == 255) {
+        uint64_t u[8];
+#pragma unroll
+        for (int g = 0; g < 8; g++) {
+          u[g] = ubg[g];
+          ubg[g] = 0;
+        }
+        cellv += group_reduce8(u, l);
+        nch = 0;
+        if (qn != q) {  // the cell is done
+          tot += cellv;
+          if ((l & 7) == 0) cb[q * 8 + (l >> 3)] = cellv;
+          cellv = 0;
+        }
+      }
+      q = qn;
+      s = sn;
+    }
+    if ((l & 7) == 0) cub[((int64_t)nt * 27 + t) * 8 + (l >> 3)] = tot;
+    unsigned long long ubmax = (l >> 3) < ngr ? tot : 0ull;
+    for (int o = 32; o > 0; o >>= 1) {
+      const unsigned long long v = __shfl_xor(ubmax, o, 64);
+      ubmax = v > ubmax ? v : ubmax;
+    }
+    if (l == 0) {
+      ubt[t] = ubmax;
+      atomicMax(&hpil[tk.x], ubmax);
+    }
+  }
 }
 
 // One wave per task: up to 64 candidates of one cell against the 27 cells
@@ -645,14 +863,11 @@ __device__ inline uint32_t sterm_bound(float g2, const SodCfg &c) {
 // neighbour chunks are compact; a chunk whose box lies beyond the reach of
 // the candidates' box is skipped.
 //
-// Branch and bound, exact, in three launches.  Phase 0 bounds every task:
-// each candidate's S is at most the sum over groups of 8 neighbours of the
-// group count times the term at the candidate-to-group-box distance; the
-// task's bound (ubt) and each halo's largest bound (hpil) are stored.
-// Phase 1 evaluates the tasks holding their halo's largest bound exactly and
-// folds their S into hbest.  Phase 2 evaluates the other tasks whose bound
-// reaches hbest and skips the rest: a skipped task can neither hold the
-// maximum nor tie with it.
+// Branch and bound, exact, in three launches.  Phase 0 (k_centre_bound)
+// bounds every task.  Phase 1 evaluates the tasks holding their halo's
+// largest bound exactly and folds their S into hbest.  Phase 2 evaluates the
+// other tasks whose bound reaches hbest and skips the rest: a skipped task
+// can neither hold the maximum nor tie with it.
 __global__ void __launch_bounds__(256)
 k_centre_task(SodCfg c, const int32_t *__restrict__ cell_start,
               const float4 *__restrict__ rec, const HaloPlan *__restrict__ plan,
@@ -665,12 +880,14 @@ k_centre_task(SodCfg c, const int32_t *__restrict__ cell_start,
               unsigned long long *__restrict__ cub,
               unsigned long long *__restrict__ cstat) {
   __shared__ float4 nbuf[4][64];  // one neighbour chunk per wave
-  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __shared__ unsigned long long cbs[4][27 * 8];  // bounds per (cell, group)
+  const int l = threadIdx.x & 63,
+            w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int32_t nt = *ntask_total;
   const float cut = c.rs2f * 1.0001f + 1e-6f;  // conservative chunk culling
   for (int32_t t = blockIdx.x * 4 + w; t < nt; t += gridDim.x * 4) {
     const int4 tk = tasks[t];
-    if (phase > 0) {
+    {
       const unsigned long long ubv = ubt[t], pil = hpil[tk.x];
       if ((phase == 1) != (ubv == pil)) continue;
       if (phase == 2) {
@@ -687,15 +904,6 @@ k_centre_task(SodCfg c, const int32_t *__restrict__ cell_start,
       }
     }
     const HaloPlan pl = plan[large[tk.x]];
-    if (phase == 0 && pl.n5 < BB_MIN) {
-      // few particles around the seed: every task of the halo is evaluated
-      // exactly in phase 1 (an all-ones bound makes each one a pilot)
-      if (l == 0) {
-        ubt[t] = ~0ull;
-        atomicMax(&hpil[tk.x], ~0ull);
-      }
-      continue;
-    }
     int ci, cj, ck;
     walk_off(tk.y, 1, &ci, &cj, &ck);
     int32_t lc[3];
@@ -717,181 +925,117 @@ k_centre_task(SodCfg c, const int32_t *__restrict__ cell_start,
     cb1[0] = wave_maxf(have ? px : -INFINITY);
     cb1[1] = wave_maxf(have ? py : -INFINITY);
     cb1[2] = wave_maxf(have ? pz : -INFINITY);
-    if (phase == 0) {
-      // boxes of the 8 groups of 8 candidates (uniform), then every
-      // neighbour (one per lane) adds the term at its distance to each box:
-      // ubg[g] bounds S of every candidate of group g
-      float g0x = have ? px : INFINITY, g0y = have ? py : INFINITY,
-            g0z = have ? pz : INFINITY;
-      float g1x = have ? px : -INFINITY, g1y = have ? py : -INFINITY,
-            g1z = have ? pz : -INFINITY;
-#pragma unroll
-      for (int o = 1; o < 8; o <<= 1) {
-        g0x = fminf(g0x, __shfl_xor(g0x, o, 64));
-        g0y = fminf(g0y, __shfl_xor(g0y, o, 64));
-        g0z = fminf(g0z, __shfl_xor(g0z, o, 64));
-        g1x = fmaxf(g1x, __shfl_xor(g1x, o, 64));
-        g1y = fmaxf(g1y, __shfl_xor(g1y, o, 64));
-        g1z = fmaxf(g1z, __shfl_xor(g1z, o, 64));
-      }
-      float B[8][6];
-#pragma unroll
-      for (int g = 0; g < 8; g++) {
-        B[g][0] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g0x), 8 * g));
-        B[g][1] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g0y), 8 * g));
-        B[g][2] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g0z), 8 * g));
-        B[g][3] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g1x), 8 * g));
-        B[g][4] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g1y), 8 * g));
-        B[g][5] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g1z), 8 * g));
-      }
-      const int ngr = (tk.w + 7) >> 3;  // groups holding candidates
-      uint64_t ubt8[8];  // the task's bound per group, all cells
-#pragma unroll
-      for (int g = 0; g < 8; g++) ubt8[g] = 0;
-      unsigned long long *cb = cub + (int64_t)t * 27 * 8;
-      for (int q = 0; q < 27; q++) {
-        int ei, ej, ek;
-        walk_off(q, 1, &ei, &ej, &ek);
-        const int di = ci + ei, dj = cj + ej, dk = ck + ek;
-        {
-          const float nb0[3] = {(float)di, (float)dj, (float)dk};
-          const float nb1[3] = {(float)di + 1.f, (float)dj + 1.f,
-                                (float)dk + 1.f};
-          if (box_gap2(cb0, cb1, nb0, nb1) > cut) {
-            if (l < 8) cb[q * 8 + l] = 0;
-            continue;
-          }
-        }
-        uint64_t ubg[8];  // this cell's part
-#pragma unroll
-        for (int g = 0; g < 8; g++) ubg[g] = 0;
-        int32_t ln[3];
-        const int32_t key = key_at(c.nx, pl.cs, di, dj, dk, ln);
-        const int32_t a = cell_start[key], b = cell_start[key + 1];
-        for (int32_t s0 = a; s0 < b; s0 += 64) {
-          const int32_t s = s0 + l;
-          float vx = 0.f, vy = 0.f, vz = 0.f;
-          bool near = false;
-          if (s < b) {
-            const float4 r = rec[s];
-            vx = rel1(r.x, ln[0], di, 0.0f);
-            vy = rel1(r.y, ln[1], dj, 0.0f);
-            vz = rel1(r.z, ln[2], dk, 0.0f);
-            // within reach of the task's box (its groups lie inside it)
-            const float ex = fmaxf(0.0f, fmaxf(cb0[0] - vx, vx - cb1[0]));
-            const float ey = fmaxf(0.0f, fmaxf(cb0[1] - vy, vy - cb1[1]));
-            const float ez = fmaxf(0.0f, fmaxf(cb0[2] - vz, vz - cb1[2]));
-            near = ex * ex + ey * ey + ez * ez <= cut;
-          }
-          if (__ballot(near) == 0) continue;  // the whole chunk is beyond reach
-          if (!near) continue;
-#pragma unroll
-          for (int g = 0; g < 8; g++) {
-            if (g < ngr) {
-              const float gx = fmaxf(0.0f, fmaxf(B[g][0] - vx, vx - B[g][3]));
-              const float gy = fmaxf(0.0f, fmaxf(B[g][1] - vy, vy - B[g][4]));
-              const float gz = fmaxf(0.0f, fmaxf(B[g][2] - vz, vz - B[g][5]));
-              ubg[g] += sterm_bound(gx * gx + gy * gy + gz * gz, c);
-            }
-          }
-        }
-#pragma unroll
-        for (int g = 0; g < 8; g++) {
-          const uint64_t v = g < ngr ? wave_sum(ubg[g]) : 0;
-          ubt8[g] += v;
-          if (l == g) cb[q * 8 + g] = v;
-        }
-      }
-      uint64_t ubmax = 0;
-#pragma unroll
-      for (int g = 0; g < 8; g++) {
-        if (l == g) cub[((int64_t)nt * 27 + t) * 8 + g] = ubt8[g];
-        if (g < ngr) ubmax = ubt8[g] > ubmax ? ubt8[g] : ubmax;
-      }
-      if (l == 0) {
-        ubt[t] = ubmax;
-        atomicMax(&hpil[tk.x], (unsigned long long)ubmax);
-      }
-      continue;
-    }
     uint64_t acc = 0;
     uint64_t npair = 0;  // neighbours tested per candidate
-    // bound of the cells not yet summed, for this lane's group
-    const unsigned long long *cb = cub + (int64_t)t * 27 * 8;
-    unsigned long long rem =
-        phase == 2 ? cub[((int64_t)nt * 27 + t) * 8 + (l >> 3)] : 0ull;
+    // record ranges of the 27 cells in nearest-first order, one per lane, all
+    // loaded at once; a cell beyond reach of the candidates' box is empty
+    int32_t qa = 0, qb = 0;
+    if (l < 27) {
+      int ei, ej, ek;
+      walk_off(kCellOrder[l], 1, &ei, &ej, &ek);
+      const int di = ci + ei, dj = cj + ej, dk = ck + ek;
+      const float nb0[3] = {(float)di, (float)dj, (float)dk};
+      const float nb1[3] = {(float)di + 1.f, (float)dj + 1.f, (float)dk + 1.f};
+      if (box_gap2(cb0, cb1, nb0, nb1) <= cut) {
+        int32_t ln[3];
+        const int32_t key = key_at(c.nx, pl.cs, di, dj, dk, ln);
+        qa = cell_start[key];
+        qb = cell_start[key + 1];
+      }
+    }
+    // bound of the cells not yet summed, for this lane's group; the bounds
+    // per (cell, group) are staged in LDS
+    unsigned long long rem = 0, best = 0;
+    if (phase == 2) {
+      const unsigned long long *cb = cub + (int64_t)t * 27 * 8;
+      for (int i = l; i < 27 * 8; i += 64) cbs[w][i] = cb[i];
+      rem = cub[((int64_t)nt * 27 + t) * 8 + (l >> 3)];
+      best = __hip_atomic_load(&hbest[tk.x], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
     bool aborted = false;
-    for (int qi = 0; qi < 27; qi++) {
+    uint64_t rest = __ballot(l < 27 && qa < qb);  // cells with records
+    // chunks of 64 records of the cells in turn (the candidates' own cell is
+    // first); the next chunk is requested before the current one is used
+    int qi = rest ? __ffsll((unsigned long long)rest) - 1 : 27;
+    int32_t s = qi < 27 ? __builtin_amdgcn_readlane(qa, qi) : 0;
+    float4 rn = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (qi < 27 && s + l < __builtin_amdgcn_readlane(qb, qi)) rn = rec[s + l];
+    bool newcell = true, first = true;
+    while (qi < 27) {
       const int q = kCellOrder[qi];
-      if (phase == 2 && qi > 0) {
+      if (newcell && phase == 2) {
         // exact sums so far plus the bound of the remaining cells
-        const unsigned long long ubc = have ? acc + rem : 0ull;
-        unsigned long long m = ubc;
-        for (int o = 32; o > 0; o >>= 1) {
-          const unsigned long long v = __shfl_xor(m, o, 64);
-          m = v > m ? v : m;
-        }
-        const unsigned long long best = __hip_atomic_load(
-            &hbest[tk.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (m < best) {
+        if (!first && __ballot(have && acc + rem >= best) == 0) {
           aborted = true;
           break;
         }
+        best = __hip_atomic_load(&hbest[tk.x], __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
+        rem -= cbs[w][q * 8 + (l >> 3)];
       }
-      if (phase == 2) rem -= cb[q * 8 + (l >> 3)];
+      first = false;
+      const int32_t b = __builtin_amdgcn_readlane(qb, qi);
+      const int32_t m = min(64, b - s);
+      const float4 r = rn;
+      int qn = qi;
+      int32_t sn = s + 64;
+      if (sn >= b) {
+        rest &= ~(1ull << qi);
+        qn = rest ? __ffsll((unsigned long long)rest) - 1 : 27;
+        sn = qn < 27 ? __builtin_amdgcn_readlane(qa, qn) : 0;
+      }
+      if (qn < 27 && sn + l < __builtin_amdgcn_readlane(qb, qn)) rn = rec[sn + l];
+      newcell = qn != qi;
+      qi = qn;
+      s = sn;
       int ei, ej, ek;
       walk_off(q, 1, &ei, &ej, &ek);
       const int di = ci + ei, dj = cj + ej, dk = ck + ek;
-      {
-        const float nb0[3] = {(float)di, (float)dj, (float)dk};
-        const float nb1[3] = {(float)di + 1.f, (float)dj + 1.f, (float)dk + 1.f};
+      // block-local coordinates of the walked cell (8 divides nx)
+      const int32_t lx = (pl.cs[0] + di) & 7, ly = (pl.cs[1] + dj) & 7,
+                    lz = (pl.cs[2] + dk) & 7;
+      float vx = 0.f, vy = 0.f, vz = 0.f;
+      if (l < m) {
+        vx = rel1(r.x, lx, di, 0.0f);
+        vy = rel1(r.y, ly, dj, 0.0f);
+        vz = rel1(r.z, lz, dk, 0.0f);
+      }
+      if (m >= 16) {
+        float nb0[3], nb1[3];
+        nb0[0] = wave_minf(l < m ? vx : INFINITY);
+        nb0[1] = wave_minf(l < m ? vy : INFINITY);
+        nb0[2] = wave_minf(l < m ? vz : INFINITY);
+        nb1[0] = wave_maxf(l < m ? vx : -INFINITY);
+        nb1[1] = wave_maxf(l < m ? vy : -INFINITY);
+        nb1[2] = wave_maxf(l < m ? vz : -INFINITY);
         if (box_gap2(cb0, cb1, nb0, nb1) > cut) continue;
       }
-      int32_t ln[3];
-      const int32_t key = key_at(c.nx, pl.cs, di, dj, dk, ln);
-      const int32_t a = cell_start[key], b = cell_start[key + 1];
-      for (int32_t s = a; s < b; s += 64) {
-        const int32_t m = min(64, b - s);
-        float vx = 0.f, vy = 0.f, vz = 0.f;
-        if (l < m) {
-          const float4 r = rec[s + l];
-          vx = rel1(r.x, ln[0], di, 0.0f);
-          vy = rel1(r.y, ln[1], dj, 0.0f);
-          vz = rel1(r.z, ln[2], dk, 0.0f);
-        }
-        if (m >= 16) {
-          float nb0[3], nb1[3];
-          nb0[0] = wave_minf(l < m ? vx : INFINITY);
-          nb0[1] = wave_minf(l < m ? vy : INFINITY);
-          nb0[2] = wave_minf(l < m ? vz : INFINITY);
-          nb1[0] = wave_maxf(l < m ? vx : -INFINITY);
-          nb1[1] = wave_maxf(l < m ? vy : -INFINITY);
-          nb1[2] = wave_maxf(l < m ? vz : -INFINITY);
-          if (box_gap2(cb0, cb1, nb0, nb1) > cut) continue;
-        }
-        uint32_t part = 0;  // <= 64 terms of <= 2^24
-        // the chunk goes through this wave's LDS slot and is read back with
-        // wave-uniform addresses (broadcast reads, no lane moves)
-        nbuf[w][l] = make_float4(vx, vy, vz, 0.0f);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // m is uniform: a scalar trip count, and a select instead of a branch
-        const int32_t mu = __builtin_amdgcn_readfirstlane(m);
+      uint32_t part = 0;  // <= 64 terms of <= 2^24
+      // the chunk goes through this wave's LDS slot and is read back with
+      // wave-uniform addresses (broadcast reads, no lane moves)
+      nbuf[w][l] = make_float4(vx, vy, vz, 0.0f);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      // m is uniform: a scalar trip count, and a select instead of a branch
+      const int32_t mu = __builtin_amdgcn_readfirstlane(m);
 #pragma unroll 4
-        for (int32_t u = 0; u < mu; u++) {
-          const float4 qv = nbuf[w][u];
-          const float dx = qv.x - px, dy = qv.y - py, dz = qv.z - pz;
-          const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-          const uint32_t term = sterm_fix(d2, c.eps2f);
-          part += d2 <= c.rs2f ? term : 0u;
-        }
-        acc += part;
-        npair += (uint64_t)m;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      for (int32_t u = 0; u < mu; u++) {
+        const float4 qv = nbuf[w][u];
+        const float dx = qv.x - px, dy = qv.y - py, dz = qv.z - pz;
+        const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        const uint32_t term = sterm_fix(d2, c.eps2f);
+        part += d2 <= c.rs2f ? term : 0u;
       }
+      acc += part;
+      npair += (uint64_t)m;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     if (aborted) {
       if (l == 0) {
@@ -1789,12 +1933,16 @@ extern "C" int psod_sod_fof(psod_ctx *ctx, const psod_sod_params *p,
     unsigned long long *hpil = cstat + 4;
     PSOD_TRY(psod_fill_u64(ctx, (uint64_t *)hbest, 2 * (int64_t)nlarge + 4, 0));
     PSOD_HIP(hipEventRecord(ctx->ev[0], st));
-    for (int phase = 0; phase < 3; phase++) {
-      k_centre_task<<<grid_for((int64_t)nt, 4, (int64_t)ctx->num_cus * 16),
-                      256, 0, st>>>(c, cat->cell_start,
-                                    (const float4 *)cat->rec, plan, large,
-                                    tasks, ntot, tres, phase, hbest, hpil,
-                                    ubt, cub, cstat);
+    const unsigned nwg = grid_for((int64_t)nt, 4, (int64_t)ctx->num_cus * 16);
+    k_centre_bound<<<nwg, 256, 0, st>>>(c, cat->cell_start,
+                                        (const float4 *)cat->rec, plan, large,
+                                        tasks, ntot, hpil, ubt, cub);
+    PSOD_LAUNCH_CHECK();
+    for (int phase = 1; phase < 3; phase++) {
+      k_centre_task<<<nwg, 256, 0, st>>>(c, cat->cell_start,
+                                         (const float4 *)cat->rec, plan, large,
+                                         tasks, ntot, tres, phase, hbest, hpil,
+                                         ubt, cub, cstat);
       PSOD_LAUNCH_CHECK();
     }
     k_centre_reduce<<<grid_for(nlarge, 256), 256, 0, st>>>(plan, large,
