@@ -650,6 +650,14 @@ __device__ inline uint32_t sterm_bound(float g2, float eps2) {
                     8.5f);
 }
 
+// Next block of work of a wave from a shared counter (the first active
+// lane takes it; every lane gets it): waves that finish early take more.
+__device__ inline int32_t wave_next(int32_t *ctr) {
+  int32_t v = 0;
+  if ((threadIdx.x & 63) == 0) v = atomicAdd(ctr, 1);
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
 // Sums over the wave of u[g] for 8 groups, transposed: lane l ends with the
 // sum of group l >> 3.  Each exchange step hands half of the values still
 // held to the partner lane (10 64-bit exchanges instead of 8 x 6).
@@ -686,383 +694,641 @@ k_centre_bound(SodCfg c, const int32_t *__restrict__ cell_start,
                const int32_t *__restrict__ ntask_total,
                unsigned long long *__restrict__ hpil,
                unsigned long long *__restrict__ ubt,
-               unsigned long long *__restrict__ cub) {
+               unsigned long long *__restrict__ cub, int32_t *__restrict__ wq) {
   // the wave index is uniform: task fields and everything derived from
   // them go to scalar registers
   const int l = threadIdx.x & 63,
             w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int32_t nt = *ntask_total;
   const float cut = c.rs2f * 1.0001f + 1e-6f;  // conservative chunk culling
-  for (int32_t t = blockIdx.x * 4 + w; t < nt; t += gridDim.x * 4) {
-    const int4 tk = tasks[t];
-    const HaloPlan pl = plan[large[tk.x]];
-    if (pl.n5 < BB_MIN) {
-      // few particles around the seed: every task of the halo is evaluated
-      // exactly in phase 1 (an all-ones bound makes each one a pilot)
-      if (l == 0) {
-        ubt[t] = ~0ull;
-        atomicMax(&hpil[tk.x], ~0ull);
+  for (;;) {
+    const int32_t t0 = wave_next(wq) * 2;
+    if (t0 >= nt) break;
+    const int32_t t1 = min(nt, t0 + 2);
+    for (int32_t t = t0; t < t1; t++) {
+      const int4 tk = tasks[t];
+      const HaloPlan pl = plan[large[tk.x]];
+      if (pl.n5 < BB_MIN) {
+        // few particles around the seed: every task of the halo is evaluated
+        // exactly in phase 1 (an all-ones bound makes each one a pilot)
+        if (l == 0) {
+          ubt[t] = ~0ull;
+          atomicMax(&hpil[tk.x], ~0ull);
+        }
+        continue;
       }
-      continue;
-    }
-    int ci, cj, ck;
-    walk_off(tk.y, 1, &ci, &cj, &ck);
-    int32_t lc[3];
-    key_at(c.nx, pl.cs, ci, cj, ck, lc);
-    float px = 0.f, py = 0.f, pz = 0.f;
-    const bool have = l < tk.w;
-    if (have) {
-      const float4 r = rec[tk.z + l];
-      px = rel1(r.x, lc[0], ci, 0.0f);
-      py = rel1(r.y, lc[1], cj, 0.0f);
-      pz = rel1(r.z, lc[2], ck, 0.0f);
-    }
-    float cb0[3], cb1[3];
-    cb0[0] = wave_minf(have ? px : INFINITY);
-    cb0[1] = wave_minf(have ? py : INFINITY);
-    cb0[2] = wave_minf(have ? pz : INFINITY);
-    cb1[0] = wave_maxf(have ? px : -INFINITY);
-    cb1[1] = wave_maxf(have ? py : -INFINITY);
-    cb1[2] = wave_maxf(have ? pz : -INFINITY);
-    // boxes of the 8 groups of 8 candidates (uniform)
-    float g0x = have ? px : INFINITY, g0y = have ? py : INFINITY,
-          g0z = have ? pz : INFINITY;
-    float g1x = have ? px : -INFINITY, g1y = have ? py : -INFINITY,
-          g1z = have ? pz : -INFINITY;
-#pragma unroll
-    for (int o = 1; o < 8; o <<= 1) {
-      g0x = fminf(g0x, __shfl_xor(g0x, o, 64));
-      g0y = fminf(g0y, __shfl_xor(g0y, o, 64));
-      g0z = fminf(g0z, __shfl_xor(g0z, o, 64));
-      g1x = fmaxf(g1x, __shfl_xor(g1x, o, 64));
-      g1y = fmaxf(g1y, __shfl_xor(g1y, o, 64));
-      g1z = fmaxf(g1z, __shfl_xor(g1z, o, 64));
-    }
-    float B[8][6];
-#pragma unroll
-    for (int g = 0; g < 8; g++) {
-      B[g][0] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g0x), 8 * g));
-      B[g][1] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g0y), 8 * g));
-      B[g][2] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g0z), 8 * g));
-      B[g][3] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g1x), 8 * g));
-      B[g][4] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g1y), 8 * g));
-      B[g][5] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g1z), 8 * g));
-    }
-    const int ngr = (tk.w + 7) >> 3;  // groups holding candidates
-    // record ranges of the 27 cells, one per lane, all loaded at once; a
-    // cell beyond reach of the candidates' box gets an empty range
-    int32_t qa = 0, qb = 0;
-    if (l < 27) {
-      int ei, ej, ek;
-      walk_off(l, 1, &ei, &ej, &ek);
-      const int di = ci + ei, dj = cj + ej, dk = ck + ek;
-      const float nb0[3] = {(float)di, (float)dj, (float)dk};
-      const float nb1[3] = {(float)di + 1.f, (float)dj + 1.f, (float)dk + 1.f};
-      if (box_gap2(cb0, cb1, nb0, nb1) <= cut) {
-        int32_t ln[3];
-        const int32_t key = key_at(c.nx, pl.cs, di, dj, dk, ln);
-        qa = cell_start[key];
-        qb = cell_start[key + 1];
+      int ci, cj, ck;
+      walk_off(tk.y, 1, &ci, &cj, &ck);
+      int32_t lc[3];
+      key_at(c.nx, pl.cs, ci, cj, ck, lc);
+      float px = 0.f, py = 0.f, pz = 0.f;
+      const bool have = l < tk.w;
+      if (have) {
+        const float4 r = rec[tk.z + l];
+        px = rel1(r.x, lc[0], ci, 0.0f);
+        py = rel1(r.y, lc[1], cj, 0.0f);
+        pz = rel1(r.z, lc[2], ck, 0.0f);
       }
-    }
-    unsigned long long *cb = cub + (int64_t)t * 27 * 8;
-    uint64_t rest = __ballot(l < 27 && qa < qb);  // cells with records
-    for (int i = l; i < 27 * 8; i += 64)
-      if (!((rest >> (i >> 3)) & 1)) cb[i] = 0;
-    uint64_t tot = 0;    // bound of group l >> 3 over the cells done
-    uint64_t cellv = 0;  // the same for the current cell, flushed part
-    // this lane's neighbours of the current cell, per group: at most 255
-    // terms of at most 2^24 + 8 each before a flush
-    uint32_t ubg[8];
-#pragma unroll
-    for (int g = 0; g < 8; g++) ubg[g] = 0;
-    int nch = 0;  // chunks summed into ubg
-    // chunks of 64 records of the cells in turn; the next chunk is requested
-    // before the current one is used
-    int q = rest ? __ffsll((unsigned long long)rest) - 1 : 27;
-    int32_t s = q < 27 ? __builtin_amdgcn_readlane(qa, q) : 0;
-    float4 rn = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (q < 27 && s + l < __builtin_amdgcn_readlane(qb, q)) rn = rec[s + l];
-    while (q < 27) {
-      const int32_t b = __builtin_amdgcn_readlane(qb, q);
-      int ei, ej, ek;
-      walk_off(q, 1, &ei, &ej, &ek);
-      const int di = ci + ei, dj = cj + ej, dk = ck + ek;
-      const float4 r = rn;
-      const bool valid = s + l < b;
-      int qn = q;
-      int32_t sn = s + 64;
-      if (sn >= b) {
-        rest &= ~(1ull << q);
-        qn = rest ? __ffsll((unsigned long long)rest) - 1 : 27;
-        sn = qn < 27 ? __builtin_amdgcn_readlane(qa, qn) : 0;
+      float cb0[3], cb1[3];
+      cb0[0] = wave_minf(have ? px : INFINITY);
+      cb0[1] = wave_minf(have ? py : INFINITY);
+      cb0[2] = wave_minf(have ? pz : INFINITY);
+      cb1[0] = wave_maxf(have ? px : -INFINITY);
+      cb1[1] = wave_maxf(have ? py : -INFINITY);
+      cb1[2] = wave_maxf(have ? pz : -INFINITY);
+      // boxes of the 8 groups of 8 candidates (uniform)
+      float g0x = have ? px : INFINITY, g0y = have ? py : INFINITY,
+            g0z = have ? pz : INFINITY;
+      float g1x = have ? px : -INFINITY, g1y = have ? py : -INFINITY,
+            g1z = have ? pz : -INFINITY;
+  #pragma unroll
+      for (int o = 1; o < 8; o <<= 1) {
+        g0x = fminf(g0x, __shfl_xor(g0x, o, 64));
+        g0y = fminf(g0y, __shfl_xor(g0y, o, 64));
+        g0z = fminf(g0z, __shfl_xor(g0z, o, 64));
+        g1x = fmaxf(g1x, __shfl_xor(g1x, o, 64));
+        g1y = fmaxf(g1y, __shfl_xor(g1y, o, 64));
+        g1z = fmaxf(g1z, __shfl_xor(g1z, o, 64));
       }
-      if (qn < 27 && sn + l < __builtin_amdgcn_readlane(qb, qn)) rn = rec[sn + l];
-      // block-local coordinates of the walked cell (8 divides nx)
-      const int32_t lx = (pl.cs[0] + di) & 7, ly = (pl.cs[1] + dj) & 7,
-                    lz = (pl.cs[2] + dk) & 7;
-      float vx = 0.f, vy = 0.f, vz = 0.f;
-      bool near = false;
-      if (valid) {
-        vx = rel1(r.x, lx, di, 0.0f);
-        vy = rel1(r.y, ly, dj, 0.0f);
-        vz = rel1(r.z, lz, dk, 0.0f);
-        // within reach of the task's box (its groups lie inside it)
-        const float ex = fmaxf(0.0f, fmaxf(cb0[0] - vx, vx - cb1[0]));
-        const float ey = fmaxf(0.0f, fmaxf(cb0[1] - vy, vy - cb1[1]));
-        const float ez = fmaxf(0.0f, fmaxf(cb0[2] - vz, vz - cb1[2]));
-        near = ex * ex + ey * ey + ez * ez <= cut;
+      float B[8][6];
+  #pragma unroll
+      for (int g = 0; g < 8; g++) {
+        B[g][0] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g0x), 8 * g));
+        B[g][1] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g0y), 8 * g));
+        B[g][2] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g0z), 8 * g));
+        B[g][3] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g1x), 8 * g));
+        B[g][4] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g1y), 8 * g));
+        B[g][5] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g1z), 8 * g));
       }
-      if (near) {
-#pragma unroll
-        for (int g = 0; g < 8; g++) {
-          if (g < ngr) {
-            // distance to the box per axis: the coordinate minus its clamp
-            const float gx = vx - __builtin_amdgcn_fmed3f(vx, B[g][0], B[g][3]);
-            const float gy = vy - __builtin_amdgcn_fmed3f(vy, B[g][1], B[g][4]);
-            const float gz = vz - __builtin_amdgcn_fmed3f(vz, B[g][2], B[g][5]);
-            const float g2 = fmaf(gx, gx, fmaf(gy, gy, gz * gz));
-            ubg[g] += g2 <= cut ? sterm_bound(g2, c.eps2f) : 0u;
+      const int ngr = (tk.w + 7) >> 3;  // groups holding candidates
+      // record ranges of the 27 cells, one per lane, all loaded at once; a
+      // cell beyond reach of the candidates' box gets an empty range
+      int32_t qa = 0, qb = 0;
+      if (l < 27) {
+        int ei, ej, ek;
+        walk_off(l, 1, &ei, &ej, &ek);
+        const int di = ci + ei, dj = cj + ej, dk = ck + ek;
+        const float nb0[3] = {(float)di, (float)dj, (float)dk};
+        const float nb1[3] = {(float)di + 1.f, (float)dj + 1.f, (float)dk + 1.f};
+        if (box_gap2(cb0, cb1, nb0, nb1) <= cut) {
+          int32_t ln[3];
+          const int32_t key = key_at(c.nx, pl.cs, di, dj, dk, ln);
+          qa = cell_start[key];
+          qb = cell_start[key + 1];
+        }
+      }
+      unsigned long long *cb = cub + (int64_t)t * 27 * 8;
+      uint64_t rest = __ballot(l < 27 && qa < qb);  // cells with records
+      for (int i = l; i < 27 * 8; i += 64)
+        if (!((rest >> (i >> 3)) & 1)) cb[i] = 0;
+      uint64_t tot = 0;    // bound of group l >> 3 over the cells done
+      uint64_t cellv = 0;  // the same for the current cell, flushed part
+      // this lane's neighbours of the current cell, per group: at most 255
+      // terms of at most 2^24 + 8 each before a flush
+      uint32_t ubg[8];
+  #pragma unroll
+      for (int g = 0; g < 8; g++) ubg[g] = 0;
+      int nch = 0;  // chunks summed into ubg
+      // chunks of 64 records of the cells in turn; the next chunk is requested
+      // before the current one is used
+      int q = rest ? __ffsll((unsigned long long)rest) - 1 : 27;
+      int32_t s = q < 27 ? __builtin_amdgcn_readlane(qa, q) : 0;
+      float4 rn = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (q < 27 && s + l < __builtin_amdgcn_readlane(qb, q)) rn = rec[s + l];
+      while (q < 27) {
+        const int32_t b = __builtin_amdgcn_readlane(qb, q);
+        int ei, ej, ek;
+        walk_off(q, 1, &ei, &ej, &ek);
+        const int di = ci + ei, dj = cj + ej, dk = ck + ek;
+        const float4 r = rn;
+        const bool valid = s + l < b;
+        int qn = q;
+        int32_t sn = s + 64;
+        if (sn >= b) {
+          rest &= ~(1ull << q);
+          qn = rest ? __ffsll((unsigned long long)rest) - 1 : 27;
+          sn = qn < 27 ? __builtin_amdgcn_readlane(qa, qn) : 0;
+        }
+        if (qn < 27 && sn + l < __builtin_amdgcn_readlane(qb, qn)) rn = rec[sn + l];
+        // block-local coordinates of the walked cell (8 divides nx)
+        const int32_t lx = (pl.cs[0] + di) & 7, ly = (pl.cs[1] + dj) & 7,
+                      lz = (pl.cs[2] + dk) & 7;
+        float vx = 0.f, vy = 0.f, vz = 0.f;
+        bool near = false;
+        if (valid) {
+          vx = rel1(r.x, lx, di, 0.0f);
+          vy = rel1(r.y, ly, dj, 0.0f);
+          vz = rel1(r.z, lz, dk, 0.0f);
+          // within reach of the task's box (its groups lie inside it)
+          const float ex = fmaxf(0.0f, fmaxf(cb0[0] - vx, vx - cb1[0]));
+          const float ey = fmaxf(0.0f, fmaxf(cb0[1] - vy, vy - cb1[1]));
+          const float ez = fmaxf(0.0f, fmaxf(cb0[2] - vz, vz - cb1[2]));
+          near = ex * ex + ey * ey + ez * ez <= cut;
+        }
+        if (near) {
+  #pragma unroll
+          for (int g = 0; g < 8; g++) {
+            if (g < ngr) {
+              // distance to the box per axis: the coordinate minus its clamp
+              const float gx = vx - __builtin_amdgcn_fmed3f(vx, B[g][0], B[g][3]);
+              const float gy = vy - __builtin_amdgcn_fmed3f(vy, B[g][1], B[g][4]);
+              const float gz = vz - __builtin_amdgcn_fmed3f(vz, B[g][2], B[g][5]);
+              const float g2 = fmaf(gx, gx, fmaf(gy, gy, gz * gz));
+              ubg[g] += g2 <= cut ? sterm_bound(g2, c.eps2f) : 0u;
+            }
           }
         }
-      }
-      if (qn != q || ++nch == 255) {
-        uint64_t u[8];
-#pragma unroll
-        for (int g = 0; g < 8; g++) {
-          u[g] = ubg[g];
-          ubg[g] = 0;
+        if (qn != q || ++nch == 255) {
+          uint64_t u[8];
+  #pragma unroll
+          for (int g = 0; g < 8; g++) {
+            u[g] = ubg[g];
+            ubg[g] = 0;
+          }
+          cellv += group_reduce8(u, l);
+          nch = 0;
+          if (qn != q) {  // the cell is done
+            tot += cellv;
+            if ((l & 7) == 0) cb[q * 8 + (l >> 3)] = cellv;
+            cellv = 0;
+          }
         }
-        cellv += group_reduce8(u, l);
-        nch = 0;
-        if (qn != q) {  // the cell is done
-          tot += cellv;
-          if ((l & 7) == 0) cb[q * 8 + (l >> 3)] = cellv;
-          cellv = 0;
-        }
+        q = qn;
+        s = sn;
       }
-      q = qn;
-      s = sn;
-    }
-    if ((l & 7) == 0) cub[((int64_t)nt * 27 + t) * 8 + (l >> 3)] = tot;
-    unsigned long long ubmax = (l >> 3) < ngr ? tot : 0ull;
-    for (int o = 32; o > 0; o >>= 1) {
-      const unsigned long long v = __shfl_xor(ubmax, o, 64);
-      ubmax = v > ubmax ? v : ubmax;
-    }
-    if (l == 0) {
-      ubt[t] = ubmax;
-      atomicMax(&hpil[tk.x], ubmax);
+      if ((l & 7) == 0) cub[((int64_t)nt * 27 + t) * 8 + (l >> 3)] = tot;
+      unsigned long long ubmax = (l >> 3) < ngr ? tot : 0ull;
+      for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long v = __shfl_xor(ubmax, o, 64);
+        ubmax = v > ubmax ? v : ubmax;
+      }
+      if (l == 0) {
+        ubt[t] = ubmax;
+        atomicMax(&hpil[tk.x], ubmax);
+      }
     }
   }
 }
 
-// One wave per task: up to 64 candidates of one cell against the 27 cells
-// around that cell (every neighbour within 1 cell lies there).  Dense cells
-// are in Morton order of their 1/8-cell sub-cells, so candidate groups and
-// neighbour chunks are compact; a chunk whose box lies beyond the reach of
-// the candidates' box is skipped.
-//
-// Branch and bound, exact, in three launches.  Phase 0 (k_centre_bound)
-// bounds every task.  Phase 1 evaluates the tasks holding their halo's
-// largest bound exactly and folds their S into hbest.  Phase 2 evaluates the
-// other tasks whose bound reaches hbest and skips the rest: a skipped task
-// can neither hold the maximum nor tie with it.
+// Pilots, one wave per task: the 64 candidates of a task (one per lane)
+// against the 27 cells around their cell (every neighbour within 1 cell lies
+// there).  Evaluates exactly the tasks holding their halo's largest bound
+// (every task of a halo that skipped the bound) and folds their S into
+// hbest.  Dense cells are in Morton order of their 1/8-cell sub-cells, so
+// candidate groups and neighbour chunks are compact; a chunk whose box lies
+// beyond the reach of the candidates' box is skipped.
 __global__ void __launch_bounds__(256)
-k_centre_task(SodCfg c, const int32_t *__restrict__ cell_start,
-              const float4 *__restrict__ rec, const HaloPlan *__restrict__ plan,
-              const int32_t *__restrict__ large, const int4 *__restrict__ tasks,
-              const int32_t *__restrict__ ntask_total,
-              uint64_t *__restrict__ taskres, int phase,
-              unsigned long long *__restrict__ hbest,
-              unsigned long long *__restrict__ hpil,
-              unsigned long long *__restrict__ ubt,
-              unsigned long long *__restrict__ cub,
-              unsigned long long *__restrict__ cstat) {
+k_centre_pilot(SodCfg c, const int32_t *__restrict__ cell_start,
+               const float4 *__restrict__ rec, const HaloPlan *__restrict__ plan,
+               const int32_t *__restrict__ large, const int4 *__restrict__ tasks,
+               const int32_t *__restrict__ ntask_total,
+               uint64_t *__restrict__ taskres,
+               unsigned long long *__restrict__ hbest,
+               const unsigned long long *__restrict__ hpil,
+               const unsigned long long *__restrict__ ubt,
+               unsigned long long *__restrict__ cstat,
+               int32_t *__restrict__ wq) {
   __shared__ float4 nbuf[4][64];  // one neighbour chunk per wave
-  __shared__ unsigned long long cbs[4][27 * 8];  // bounds per (cell, group)
   const int l = threadIdx.x & 63,
             w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int32_t nt = *ntask_total;
   const float cut = c.rs2f * 1.0001f + 1e-6f;  // conservative chunk culling
-  for (int32_t t = blockIdx.x * 4 + w; t < nt; t += gridDim.x * 4) {
-    const int4 tk = tasks[t];
-    {
-      const unsigned long long ubv = ubt[t], pil = hpil[tk.x];
-      if ((phase == 1) != (ubv == pil)) continue;
-      if (phase == 2) {
-        const unsigned long long best = __hip_atomic_load(
-            &hbest[tk.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (ubv < best) {
-          if (l == 0) {
-            taskres[2 * (int64_t)t] = 0;
-            taskres[2 * (int64_t)t + 1] = ~0ull;
-            atomicAdd(&cstat[1], 1ull);
-          }
-          continue;
+  for (;;) {
+    const int32_t t0 = wave_next(wq) * 16;
+    if (t0 >= nt) break;
+    const int32_t t1 = min(nt, t0 + 16);
+    for (int32_t t = t0; t < t1; t++) {
+      const int4 tk = tasks[t];
+      if (ubt[t] != hpil[tk.x]) continue;
+      const HaloPlan pl = plan[large[tk.x]];
+      int ci, cj, ck;
+      walk_off(tk.y, 1, &ci, &cj, &ck);
+      int32_t lc[3];
+      key_at(c.nx, pl.cs, ci, cj, ck, lc);
+      float px = 0.f, py = 0.f, pz = 0.f;
+      int32_t pidx = 0;
+      const bool have = l < tk.w;
+      if (have) {
+        const float4 r = rec[tk.z + l];
+        px = rel1(r.x, lc[0], ci, 0.0f);
+        py = rel1(r.y, lc[1], cj, 0.0f);
+        pz = rel1(r.z, lc[2], ck, 0.0f);
+        pidx = __float_as_int(r.w);
+      }
+      float cb0[3], cb1[3];
+      cb0[0] = wave_minf(have ? px : INFINITY);
+      cb0[1] = wave_minf(have ? py : INFINITY);
+      cb0[2] = wave_minf(have ? pz : INFINITY);
+      cb1[0] = wave_maxf(have ? px : -INFINITY);
+      cb1[1] = wave_maxf(have ? py : -INFINITY);
+      cb1[2] = wave_maxf(have ? pz : -INFINITY);
+      uint64_t acc = 0;
+      uint64_t npair = 0;  // neighbours tested per candidate
+      // record ranges of the 27 cells, one per lane, all loaded at once; a cell
+      // beyond reach of the candidates' box is empty
+      int32_t qa = 0, qb = 0;
+      if (l < 27) {
+        int ei, ej, ek;
+        walk_off(l, 1, &ei, &ej, &ek);
+        const int di = ci + ei, dj = cj + ej, dk = ck + ek;
+        const float nb0[3] = {(float)di, (float)dj, (float)dk};
+        const float nb1[3] = {(float)di + 1.f, (float)dj + 1.f, (float)dk + 1.f};
+        if (box_gap2(cb0, cb1, nb0, nb1) <= cut) {
+          int32_t ln[3];
+          const int32_t key = key_at(c.nx, pl.cs, di, dj, dk, ln);
+          qa = cell_start[key];
+          qb = cell_start[key + 1];
         }
       }
-    }
-    const HaloPlan pl = plan[large[tk.x]];
-    int ci, cj, ck;
-    walk_off(tk.y, 1, &ci, &cj, &ck);
-    int32_t lc[3];
-    key_at(c.nx, pl.cs, ci, cj, ck, lc);
-    float px = 0.f, py = 0.f, pz = 0.f;
-    int32_t pidx = 0;
-    const bool have = l < tk.w;
-    if (have) {
-      const float4 r = rec[tk.z + l];
-      px = rel1(r.x, lc[0], ci, 0.0f);
-      py = rel1(r.y, lc[1], cj, 0.0f);
-      pz = rel1(r.z, lc[2], ck, 0.0f);
-      pidx = __float_as_int(r.w);
-    }
-    float cb0[3], cb1[3];
-    cb0[0] = wave_minf(have ? px : INFINITY);
-    cb0[1] = wave_minf(have ? py : INFINITY);
-    cb0[2] = wave_minf(have ? pz : INFINITY);
-    cb1[0] = wave_maxf(have ? px : -INFINITY);
-    cb1[1] = wave_maxf(have ? py : -INFINITY);
-    cb1[2] = wave_maxf(have ? pz : -INFINITY);
-    uint64_t acc = 0;
-    uint64_t npair = 0;  // neighbours tested per candidate
-    // record ranges of the 27 cells in nearest-first order, one per lane, all
-    // loaded at once; a cell beyond reach of the candidates' box is empty
-    int32_t qa = 0, qb = 0;
-    if (l < 27) {
-      int ei, ej, ek;
-      walk_off(kCellOrder[l], 1, &ei, &ej, &ek);
-      const int di = ci + ei, dj = cj + ej, dk = ck + ek;
-      const float nb0[3] = {(float)di, (float)dj, (float)dk};
-      const float nb1[3] = {(float)di + 1.f, (float)dj + 1.f, (float)dk + 1.f};
-      if (box_gap2(cb0, cb1, nb0, nb1) <= cut) {
-        int32_t ln[3];
-        const int32_t key = key_at(c.nx, pl.cs, di, dj, dk, ln);
-        qa = cell_start[key];
-        qb = cell_start[key + 1];
-      }
-    }
-    // bound of the cells not yet summed, for this lane's group; the bounds
-    // per (cell, group) are staged in LDS
-    unsigned long long rem = 0, best = 0;
-    if (phase == 2) {
-      const unsigned long long *cb = cub + (int64_t)t * 27 * 8;
-      for (int i = l; i < 27 * 8; i += 64) cbs[w][i] = cb[i];
-      rem = cub[((int64_t)nt * 27 + t) * 8 + (l >> 3)];
-      best = __hip_atomic_load(&hbest[tk.x], __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
-    bool aborted = false;
-    uint64_t rest = __ballot(l < 27 && qa < qb);  // cells with records
-    // chunks of 64 records of the cells in turn (the candidates' own cell is
-    // first); the next chunk is requested before the current one is used
-    int qi = rest ? __ffsll((unsigned long long)rest) - 1 : 27;
-    int32_t s = qi < 27 ? __builtin_amdgcn_readlane(qa, qi) : 0;
-    float4 rn = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (qi < 27 && s + l < __builtin_amdgcn_readlane(qb, qi)) rn = rec[s + l];
-    bool newcell = true, first = true;
-    while (qi < 27) {
-      const int q = kCellOrder[qi];
-      if (newcell && phase == 2) {
-        // exact sums so far plus the bound of the remaining cells
-        if (!first && __ballot(have && acc + rem >= best) == 0) {
-          aborted = true;
-          break;
+      uint64_t rest = __ballot(l < 27 && qa < qb);  // cells with records
+      // chunks of 64 records of the cells in turn; the next chunk is requested
+      // before the current one is used
+      int q = rest ? __ffsll((unsigned long long)rest) - 1 : 27;
+      int32_t s = q < 27 ? __builtin_amdgcn_readlane(qa, q) : 0;
+      float4 rn = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (q < 27 && s + l < __builtin_amdgcn_readlane(qb, q)) rn = rec[s + l];
+      while (q < 27) {
+        const int32_t b = __builtin_amdgcn_readlane(qb, q);
+        const int32_t m = min(64, b - s);
+        const float4 r = rn;
+        const int qc = q;
+        int32_t sn = s + 64;
+        if (sn >= b) {
+          rest &= ~(1ull << q);
+          q = rest ? __ffsll((unsigned long long)rest) - 1 : 27;
+          sn = q < 27 ? __builtin_amdgcn_readlane(qa, q) : 0;
         }
-        best = __hip_atomic_load(&hbest[tk.x], __ATOMIC_RELAXED,
-                                 __HIP_MEMORY_SCOPE_AGENT);
-        rem -= cbs[w][q * 8 + (l >> 3)];
+        if (q < 27 && sn + l < __builtin_amdgcn_readlane(qb, q)) rn = rec[sn + l];
+        s = sn;
+        int ei, ej, ek;
+        walk_off(qc, 1, &ei, &ej, &ek);
+        const int di = ci + ei, dj = cj + ej, dk = ck + ek;
+        // block-local coordinates of the walked cell (8 divides nx)
+        const int32_t lx = (pl.cs[0] + di) & 7, ly = (pl.cs[1] + dj) & 7,
+                      lz = (pl.cs[2] + dk) & 7;
+        float vx = 0.f, vy = 0.f, vz = 0.f;
+        if (l < m) {
+          vx = rel1(r.x, lx, di, 0.0f);
+          vy = rel1(r.y, ly, dj, 0.0f);
+          vz = rel1(r.z, lz, dk, 0.0f);
+        }
+        if (m >= 16) {
+          float nb0[3], nb1[3];
+          nb0[0] = wave_minf(l < m ? vx : INFINITY);
+          nb0[1] = wave_minf(l < m ? vy : INFINITY);
+          nb0[2] = wave_minf(l < m ? vz : INFINITY);
+          nb1[0] = wave_maxf(l < m ? vx : -INFINITY);
+          nb1[1] = wave_maxf(l < m ? vy : -INFINITY);
+          nb1[2] = wave_maxf(l < m ? vz : -INFINITY);
+          if (box_gap2(cb0, cb1, nb0, nb1) > cut) continue;
+        }
+        uint32_t part = 0;  // <= 64 terms of <= 2^24
+        // the chunk goes through this wave's LDS slot and is read back with
+        // wave-uniform addresses (broadcast reads, no lane moves)
+        nbuf[w][l] = make_float4(vx, vy, vz, 0.0f);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // m is uniform: a scalar trip count, and a select instead of a branch
+        const int32_t mu = __builtin_amdgcn_readfirstlane(m);
+  #pragma unroll 4
+        for (int32_t u = 0; u < mu; u++) {
+          const float4 qv = nbuf[w][u];
+          const float dx = qv.x - px, dy = qv.y - py, dz = qv.z - pz;
+          const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+          const uint32_t term = sterm_fix(d2, c.eps2f);
+          part += d2 <= c.rs2f ? term : 0u;
+        }
+        acc += part;
+        npair += (uint64_t)m;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       }
-      first = false;
-      const int32_t b = __builtin_amdgcn_readlane(qb, qi);
-      const int32_t m = min(64, b - s);
-      const float4 r = rn;
-      int qn = qi;
-      int32_t sn = s + 64;
-      if (sn >= b) {
-        rest &= ~(1ull << qi);
-        qn = rest ? __ffsll((unsigned long long)rest) - 1 : 27;
-        sn = qn < 27 ? __builtin_amdgcn_readlane(qa, qn) : 0;
-      }
-      if (qn < 27 && sn + l < __builtin_amdgcn_readlane(qb, qn)) rn = rec[sn + l];
-      newcell = qn != qi;
-      qi = qn;
-      s = sn;
-      int ei, ej, ek;
-      walk_off(q, 1, &ei, &ej, &ek);
-      const int di = ci + ei, dj = cj + ej, dk = ck + ek;
-      // block-local coordinates of the walked cell (8 divides nx)
-      const int32_t lx = (pl.cs[0] + di) & 7, ly = (pl.cs[1] + dj) & 7,
-                    lz = (pl.cs[2] + dk) & 7;
-      float vx = 0.f, vy = 0.f, vz = 0.f;
-      if (l < m) {
-        vx = rel1(r.x, lx, di, 0.0f);
-        vy = rel1(r.y, ly, dj, 0.0f);
-        vz = rel1(r.z, lz, dk, 0.0f);
-      }
-      if (m >= 16) {
-        float nb0[3], nb1[3];
-        nb0[0] = wave_minf(l < m ? vx : INFINITY);
-        nb0[1] = wave_minf(l < m ? vy : INFINITY);
-        nb0[2] = wave_minf(l < m ? vz : INFINITY);
-        nb1[0] = wave_maxf(l < m ? vx : -INFINITY);
-        nb1[1] = wave_maxf(l < m ? vy : -INFINITY);
-        nb1[2] = wave_maxf(l < m ? vz : -INFINITY);
-        if (box_gap2(cb0, cb1, nb0, nb1) > cut) continue;
-      }
-      uint32_t part = 0;  // <= 64 terms of <= 2^24
-      // the chunk goes through this wave's LDS slot and is read back with
-      // wave-uniform addresses (broadcast reads, no lane moves)
-      nbuf[w][l] = make_float4(vx, vy, vz, 0.0f);
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      // m is uniform: a scalar trip count, and a select instead of a branch
-      const int32_t mu = __builtin_amdgcn_readfirstlane(m);
-#pragma unroll 4
-      for (int32_t u = 0; u < mu; u++) {
-        const float4 qv = nbuf[w][u];
-        const float dx = qv.x - px, dy = qv.y - py, dz = qv.z - pz;
-        const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-        const uint32_t term = sterm_fix(d2, c.eps2f);
-        part += d2 <= c.rs2f ? term : 0u;
-      }
-      acc += part;
-      npair += (uint64_t)m;
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
-    if (aborted) {
+      uint64_t key = have ? (((uint64_t)(uint32_t)tk.y << 32) | (uint32_t)pidx)
+                          : ~0ull;
+      if (!have) acc = 0;
+      wave_argmax(acc, key);
       if (l == 0) {
-        taskres[2 * (int64_t)t] = 0;
-        taskres[2 * (int64_t)t + 1] = ~0ull;
-        atomicAdd(&cstat[3], 1ull);
+        taskres[2 * (int64_t)t] = acc;
+        taskres[2 * (int64_t)t + 1] = key;
+        atomicMax(&hbest[tk.x], (unsigned long long)acc);
+        atomicAdd(&cstat[0], 1ull);
         atomicAdd(&cstat[2], (unsigned long long)(npair * (uint64_t)tk.w));
       }
-      continue;
-    }
-    uint64_t key = have ? (((uint64_t)(uint32_t)tk.y << 32) | (uint32_t)pidx)
-                        : ~0ull;
-    if (!have) acc = 0;
-    wave_argmax(acc, key);
-    if (l == 0) {
-      taskres[2 * (int64_t)t] = acc;
-      taskres[2 * (int64_t)t + 1] = key;
-      atomicMax(&hbest[tk.x], (unsigned long long)acc);
-      atomicAdd(&cstat[0], 1ull);
-      atomicAdd(&cstat[2], (unsigned long long)(npair * (uint64_t)tk.w));
     }
   }
 }
 
+// Work list of the exact phase: item t*8 + g for every candidate group g of
+// a task that was not a pilot, whose bound reaches the halo's best after the
+// pilots.  A group whose bound is below the best can neither hold the
+// maximum nor tie with it.
+__global__ void __launch_bounds__(256)
+k_task_items(const int4 *__restrict__ tasks,
+             const int32_t *__restrict__ ntask_total,
+             const unsigned long long *__restrict__ ubt,
+             const unsigned long long *__restrict__ hpil,
+             const unsigned long long *__restrict__ cub,
+             const unsigned long long *__restrict__ hbest,
+             int32_t *__restrict__ items, int32_t *__restrict__ nitems,
+             unsigned long long *__restrict__ cstat) {
+  const int32_t nt = *ntask_total;
+  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t mask = 0;
+  bool skipped = false;
+  if (t < nt) {
+    const int4 tk = tasks[t];
+    const unsigned long long ub = ubt[t];
+    if (ub != hpil[tk.x]) {
+      const unsigned long long best = hbest[tk.x];
+      const int ngr = (tk.w + 7) >> 3;
+      if (ub >= best)
+        for (int g = 0; g < ngr; g++)
+          if (cub[((int64_t)nt * 27 + t) * 8 + g] >= best) mask |= 1u << g;
+      skipped = mask == 0;
+    }
+  }
+  int32_t tot;
+  const int32_t ex = wave_excl(__popc(mask), &tot);
+  int32_t base = 0;
+  if ((threadIdx.x & 63) == 0 && tot > 0) base = atomicAdd(nitems, tot);
+  int32_t p = __shfl(base, 0, 64) + ex;
+  while (mask) {
+    const int g = __ffs(mask) - 1;
+    mask &= mask - 1;
+    items[p++] = t * 8 + g;
+  }
+  const unsigned long long ns = wave_sum((unsigned long long)(skipped ? 1 : 0));
+  if ((threadIdx.x & 63) == 0 && ns) atomicAdd(&cstat[1], ns);
+}
+
+// Shell of the cell at position qi of the nearest-first order: the cell
+// itself, the 6 faces, the 12 edges, the 8 corners.
+__device__ inline int cell_shell(int qi) {
+  return qi == 0 ? 0 : qi < 7 ? 1 : qi < 19 ? 2 : 3;
+}
+
+// Exact phase, one wave per item (task t, group g).  Lane l takes candidate
+// l & 7 of the group.  The neighbours within reach of the group's box are
+// queued in LDS across chunks and cells and taken 64 at a time, neighbour
+// u + (l >> 3) by lane l in steps of 8, so that every step evaluates 64
+// pairs.  Each pair's term comes from the same operands and operations as in
+// the pilots (neighbour minus candidate, the same fused squares), so S is
+// the same integer.  After each shell of cells (nearest first) the queue is
+// drained and the group is abandoned when no candidate's exact sum so far
+// plus the bound of the cells left reaches the halo's best.
+__global__ void __launch_bounds__(256)
+k_centre_group(SodCfg c, const int32_t *__restrict__ cell_start,
+               const float4 *__restrict__ rec, const HaloPlan *__restrict__ plan,
+               const int32_t *__restrict__ large, const int4 *__restrict__ tasks,
+               const int32_t *__restrict__ ntask_total,
+               const int32_t *__restrict__ items,
+               const int32_t *__restrict__ nitems,
+               unsigned long long *__restrict__ hbest,
+               const unsigned long long *__restrict__ cub,
+               uint64_t *__restrict__ itemres,
+               unsigned long long *__restrict__ cstat,
+               int32_t *__restrict__ wq) {
+  __shared__ float4 gq[4][128];  // queued neighbours, one queue per wave
+  const int l = threadIdx.x & 63,
+            w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int32_t nt = *ntask_total, ni = *nitems;
+  const float cut = c.rs2f * 1.0001f + 1e-6f;  // conservative culling
+  const uint64_t below = l == 0 ? 0ull : (~0ull >> (64 - l));  // lanes < l
+  // Blocks of 2 items, visited in a strided order: block k is block
+  // (k % 32) * nb32 + k / 32 of the list, so that the items of one halo
+  // (contiguous in the list) start at times spread over the launch and the
+  // later ones meet the best S raised by the earlier ones.
+  const int32_t nblk = (ni + 1) >> 1, nb32 = (nblk + 31) >> 5;
+  for (;;) {
+    const int32_t k = wave_next(wq);
+    if (k >= 32 * nb32) break;
+    const int32_t i0 = 2 * ((k & 31) * nb32 + (k >> 5));
+    if (i0 >= ni) continue;
+    const int32_t i1 = min(ni, i0 + 2);
+    for (int32_t i = i0; i < i1; i++) {
+      const int32_t it = items[i];
+      const int32_t t = it >> 3, g = it & 7;
+      const int4 tk = tasks[t];
+      uint64_t *res = itemres + 2 * (int64_t)it;
+      unsigned long long best = __hip_atomic_load(
+          &hbest[tk.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (cub[((int64_t)nt * 27 + t) * 8 + g] < best) {
+        if (l == 0) {
+          res[0] = 0;
+          res[1] = ~0ull;
+          atomicAdd(&cstat[1], 1ull);
+        }
+        continue;
+      }
+      const HaloPlan pl = plan[large[tk.x]];
+      int ci, cj, ck;
+      walk_off(tk.y, 1, &ci, &cj, &ck);
+      int32_t lc[3];
+      key_at(c.nx, pl.cs, ci, cj, ck, lc);
+      const int nc = min(8, tk.w - 8 * g);  // candidates of the group
+      const int kc = l & 7;                 // this lane's candidate
+      // lanes of missing candidates sit beyond every cutoff
+      float px = 1e30f, py = 1e30f, pz = 1e30f;
+      int32_t pidx = 0;
+      if (kc < nc) {
+        const float4 r = rec[tk.z + 8 * g + kc];
+        px = rel1(r.x, lc[0], ci, 0.0f);
+        py = rel1(r.y, lc[1], cj, 0.0f);
+        pz = rel1(r.z, lc[2], ck, 0.0f);
+        pidx = __float_as_int(r.w);
+      }
+      float gb0[3], gb1[3];  // the group's box
+      gb0[0] = wave_minf(kc < nc ? px : INFINITY);
+      gb0[1] = wave_minf(kc < nc ? py : INFINITY);
+      gb0[2] = wave_minf(kc < nc ? pz : INFINITY);
+      gb1[0] = wave_maxf(kc < nc ? px : -INFINITY);
+      gb1[1] = wave_maxf(kc < nc ? py : -INFINITY);
+      gb1[2] = wave_maxf(kc < nc ? pz : -INFINITY);
+      // record ranges of the 27 cells nearest first and the group's bound in
+      // each, one per lane; a cell beyond reach of the group's box is empty
+      // (its bound is 0: none of its records is within reach of the group)
+      int32_t qa = 0, qb = 0, cq = 0;
+      unsigned long long qbnd = 0;
+      if (l < 27) {
+        const int q = kCellOrder[l];
+        cq = q;
+        int ei, ej, ek;
+        walk_off(q, 1, &ei, &ej, &ek);
+        const int di = ci + ei, dj = cj + ej, dk = ck + ek;
+        const float nb0[3] = {(float)di, (float)dj, (float)dk};
+        const float nb1[3] = {(float)di + 1.f, (float)dj + 1.f, (float)dk + 1.f};
+        if (box_gap2(gb0, gb1, nb0, nb1) <= cut) {
+          int32_t ln[3];
+          const int32_t key = key_at(c.nx, pl.cs, di, dj, dk, ln);
+          qa = cell_start[key];
+          qb = cell_start[key + 1];
+        }
+        qbnd = cub[(int64_t)t * 27 * 8 + q * 8 + g];
+      }
+      // bound of cells qi.. (nearest-first order): suffix sums over lanes
+  #pragma unroll
+      for (int o = 1; o < 32; o <<= 1) {
+        const unsigned long long v = __shfl_down(qbnd, o, 64);
+        if (l + o < 27) qbnd += v;
+      }
+      uint64_t rest = __ballot(l < 27 && qa < qb);  // cells with records
+      int qi = rest ? __ffsll((unsigned long long)rest) - 1 : 27;
+      int32_t s = qi < 27 ? __builtin_amdgcn_readlane(qa, qi) : 0;
+      float4 rn = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (qi < 27 && s + l < __builtin_amdgcn_readlane(qb, qi)) rn = rec[s + l];
+      // this lane's share of its candidate's sum: u32 steps of at most 2^24,
+      // folded into acc64 before 255 steps
+      uint32_t part = 0;
+      int nst = 0, qlen = 0;
+      uint64_t acc64 = 0, npair = 0;
+      bool aborted = false;
+      // n queued neighbours from the queue's head, 8 per step
+      auto drain = [&](int n) {
+        if (nst > 255 - 8) {
+          acc64 += part;
+          part = 0;
+          nst = 0;
+        }
+        nst += (n + 7) >> 3;
+        for (int u = 0; u < n; u += 8) {
+          const int j = u + (l >> 3);
+          const float4 qv = gq[w][j & 127];
+          const float dx = qv.x - px, dy = qv.y - py, dz = qv.z - pz;
+          const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+          const uint32_t term = sterm_fix(d2, c.eps2f);
+          part += (j < n && d2 <= c.rs2f) ? term : 0u;
+        }
+      };
+      while (qi < 27) {
+        const int q = __builtin_amdgcn_readlane(cq, qi);
+        const int32_t b = __builtin_amdgcn_readlane(qb, qi);
+        const int32_t m = min(64, b - s);
+        const float4 r = rn;
+        int qn = qi;
+        int32_t sn = s + 64;
+        if (sn >= b) {
+          rest &= ~(1ull << qi);
+          qn = rest ? __ffsll((unsigned long long)rest) - 1 : 27;
+          sn = qn < 27 ? __builtin_amdgcn_readlane(qa, qn) : 0;
+        }
+        if (qn < 27 && sn + l < __builtin_amdgcn_readlane(qb, qn)) rn = rec[sn + l];
+        int ei, ej, ek;
+        walk_off(q, 1, &ei, &ej, &ek);
+        const int di = ci + ei, dj = cj + ej, dk = ck + ek;
+        // block-local coordinates of the walked cell (8 divides nx)
+        const int32_t lx = (pl.cs[0] + di) & 7, ly = (pl.cs[1] + dj) & 7,
+                      lz = (pl.cs[2] + dk) & 7;
+        bool near = false;
+        float vx = 0.f, vy = 0.f, vz = 0.f;
+        if (l < m) {
+          vx = rel1(r.x, lx, di, 0.0f);
+          vy = rel1(r.y, ly, dj, 0.0f);
+          vz = rel1(r.z, lz, dk, 0.0f);
+          const float ex = fmaxf(0.0f, fmaxf(gb0[0] - vx, vx - gb1[0]));
+          const float ey = fmaxf(0.0f, fmaxf(gb0[1] - vy, vy - gb1[1]));
+          const float ez = fmaxf(0.0f, fmaxf(gb0[2] - vz, vz - gb1[2]));
+          near = ex * ex + ey * ey + ez * ez <= cut;
+        }
+        const uint64_t nm = __ballot(near);
+        const int nn = __popcll(nm);
+        if (near) gq[w][qlen + __popcll(nm & below)] = make_float4(vx, vy, vz, 0.f);
+        qlen += nn;
+        npair += (uint64_t)nn;
+        const bool shell_end = qn == 27 || cell_shell(qn) != cell_shell(qi);
+        if (qlen >= 64 || (shell_end && qlen > 0)) {
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+          const int n = qlen >= 64 ? 64 : qlen;
+          drain(n);
+          // move the entries past the drained ones to the queue's head
+          qlen -= n;
+          float4 mv = make_float4(0.f, 0.f, 0.f, 0.f);
+          if (l < qlen) mv = gq[w][64 + l];
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+          if (l < qlen) gq[w][l] = mv;
+          if (shell_end && qlen > 0) {  // at most 63 left: drain them too
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            drain(qlen);
+            qlen = 0;
+          }
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        if (shell_end && qn < 27) {
+          // every cell before qn is summed, the cells from qn on are bounded
+          acc64 += part;
+          part = 0;
+          nst = 0;
+          uint64_t v = acc64;  // sum over the 8 lanes of each candidate
+          v += __shfl_xor(v, 8, 64);
+          v += __shfl_xor(v, 16, 64);
+          v += __shfl_xor(v, 32, 64);
+          const uint32_t blo = __builtin_amdgcn_readlane((uint32_t)qbnd, qn);
+          const uint32_t bhi =
+              __builtin_amdgcn_readlane((uint32_t)(qbnd >> 32), qn);
+          const unsigned long long rem = ((unsigned long long)bhi << 32) | blo;
+          if (__ballot(l < nc && v + rem >= best) == 0) {
+            aborted = true;
+            break;
+          }
+          best = __hip_atomic_load(&hbest[tk.x], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        }
+        qi = qn;
+        s = sn;
+      }
+      if (aborted) {
+        if (l == 0) {
+          res[0] = 0;
+          res[1] = ~0ull;
+          atomicAdd(&cstat[3], 1ull);
+          atomicAdd(&cstat[2], (unsigned long long)(npair * (uint64_t)nc));
+        }
+        continue;
+      }
+      acc64 += part;
+      uint64_t tot = acc64;  // sum over the 8 lanes of each candidate
+      tot += __shfl_xor(tot, 8, 64);
+      tot += __shfl_xor(tot, 16, 64);
+      tot += __shfl_xor(tot, 32, 64);
+      // lanes 0..nc-1 hold the candidates' S
+      uint64_t acc = l < nc ? tot : 0;
+      uint64_t key = l < nc ? (((uint64_t)(uint32_t)tk.y << 32) | (uint32_t)pidx)
+                            : ~0ull;
+      wave_argmax(acc, key);
+      if (l == 0) {
+        res[0] = acc;
+        res[1] = key;
+        atomicMax(&hbest[tk.x], (unsigned long long)acc);
+        atomicAdd(&cstat[0], 1ull);
+        atomicAdd(&cstat[2], (unsigned long long)(npair * (uint64_t)nc));
+      }
+    }
+  }
+}
+
+// result pairs (S, key) that lose to any evaluated candidate
+__global__ void k_res_init(uint64_t *__restrict__ r, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  r[2 * i] = 0;
+  r[2 * i + 1] = ~0ull;
+}
+
+// Each halo's centre: the best (S, key) over its pilot tasks and the groups
+// of its other tasks, key = (candidate cell rank << 32) | particle index.
 __global__ void __launch_bounds__(256)
 k_centre_reduce(HaloPlan *__restrict__ plan, const int32_t *__restrict__ large,
-                int32_t nlarge, const uint64_t *__restrict__ taskres) {
+                int32_t nlarge, const uint64_t *__restrict__ taskres,
+                const uint64_t *__restrict__ itemres) {
   const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nlarge) return;
   const int32_t h = large[e];
@@ -1072,6 +1338,11 @@ k_centre_reduce(HaloPlan *__restrict__ plan, const int32_t *__restrict__ large,
   for (int32_t t = pl.task0; t < pl.task0 + pl.ntask; t++) {
     const uint64_t a = taskres[2 * (int64_t)t], k = taskres[2 * (int64_t)t + 1];
     if (better(a, k, ba, bk)) { ba = a; bk = k; }
+    for (int g = 0; g < 8; g++) {
+      const int64_t it = (int64_t)t * 8 + g;
+      const uint64_t a2 = itemres[2 * it], k2 = itemres[2 * it + 1];
+      if (better(a2, k2, ba, bk)) { ba = a2; bk = k2; }
+    }
   }
   plan[h].centre = key_idx(bk);
 }
@@ -1915,38 +2186,67 @@ extern "C" int psod_sod_fof(psod_ctx *ctx, const psod_sod_params *p,
   int32_t nt = 0;
   PSOD_HIP(hipMemcpyAsync(&nt, ntot, 4, hipMemcpyDeviceToHost, st));
   PSOD_HIP(hipStreamSynchronize(st));
+  if (nt > (1 << 27))
+    return psod_set_error(PSOD_E_CAPACITY,
+                          "%d centre-search tasks exceed 2^27", nt);
   if (nt > 0) {
-    // tasks, their results, bounds, and bounds per (cell, group) plus the
-    // per-group totals (28 x 8 per task)
+    // tasks, their results, bounds, bounds per (cell, group) plus the
+    // per-group totals (28 x 8 per task), the results of the groups (8 per
+    // task), the work list of groups and its length
     int4 *tasks = (int4 *)psod_scratch(
-        ctx, S_TASKRES, (size_t)nt * (sizeof(int4) + 24 + 28 * 8 * 8));
+        ctx, S_TASKRES,
+        (size_t)nt * (sizeof(int4) + 24 + 28 * 8 * 8 + 8 * 16 + 8 * 4) + 256);
     if (!tasks) return PSOD_E_NOMEM;
     uint64_t *tres = (uint64_t *)(tasks + nt);
     unsigned long long *ubt = (unsigned long long *)(tres + 2 * (int64_t)nt);
     unsigned long long *cub = ubt + nt;
+    uint64_t *ires = (uint64_t *)(cub + (int64_t)nt * 28 * 8);
+    int32_t *items = (int32_t *)(ires + 16 * (int64_t)nt);
+    int32_t *nitems = items + 8 * (int64_t)nt;
+    int32_t *wq = nitems + 4;  // work counters of the bound, pilot, group
     k_task_emit<<<grid_for(nlarge, 256), 256, 0, st>>>(
         c, cat->cell_start, plan, large, nlarge, ntask, task0, tasks);
     PSOD_LAUNCH_CHECK();
-    // hbest[nlarge..nlarge+3]: tasks evaluated, tasks skipped, pair tests;
-    // then each halo's largest task bound
+    k_res_init<<<grid_for(nt, 256), 256, 0, st>>>(tres, nt);
+    PSOD_LAUNCH_CHECK();
+    k_res_init<<<grid_for(8 * (int64_t)nt, 256), 256, 0, st>>>(
+        ires, 8 * (int64_t)nt);
+    PSOD_LAUNCH_CHECK();
+    // hbest[nlarge..nlarge+3]: units evaluated, units skipped, pair tests,
+    // units abandoned; then each halo's largest task bound
     unsigned long long *cstat = hbest + nlarge;
     unsigned long long *hpil = cstat + 4;
     PSOD_TRY(psod_fill_u64(ctx, (uint64_t *)hbest, 2 * (int64_t)nlarge + 4, 0));
+    PSOD_TRY(psod_fill_i32(ctx, nitems, 8, 0));
     PSOD_HIP(hipEventRecord(ctx->ev[0], st));
-    const unsigned nwg = grid_for((int64_t)nt, 4, (int64_t)ctx->num_cus * 16);
+    // as many workgroups as fit at once; their waves take work from counters
+    auto resident = [&](const void *k) {
+      int nb = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, 256, 0) !=
+              hipSuccess ||
+          nb < 1)
+        nb = 4;
+      return (unsigned)ctx->num_cus * (unsigned)nb;
+    };
+    const unsigned nwg = resident((const void *)k_centre_bound);
     k_centre_bound<<<nwg, 256, 0, st>>>(c, cat->cell_start,
                                         (const float4 *)cat->rec, plan, large,
-                                        tasks, ntot, hpil, ubt, cub);
+                                        tasks, ntot, hpil, ubt, cub, wq);
     PSOD_LAUNCH_CHECK();
-    for (int phase = 1; phase < 3; phase++) {
-      k_centre_task<<<nwg, 256, 0, st>>>(c, cat->cell_start,
-                                         (const float4 *)cat->rec, plan, large,
-                                         tasks, ntot, tres, phase, hbest, hpil,
-                                         ubt, cub, cstat);
-      PSOD_LAUNCH_CHECK();
-    }
-    k_centre_reduce<<<grid_for(nlarge, 256), 256, 0, st>>>(plan, large,
-                                                           nlarge, tres);
+    k_centre_pilot<<<resident((const void *)k_centre_pilot), 256, 0, st>>>(c, cat->cell_start,
+                                        (const float4 *)cat->rec, plan, large,
+                                        tasks, ntot, tres, hbest, hpil, ubt,
+                                        cstat, wq + 1);
+    PSOD_LAUNCH_CHECK();
+    k_task_items<<<grid_for(nt, 256), 256, 0, st>>>(tasks, ntot, ubt, hpil, cub,
+                                                    hbest, items, nitems, cstat);
+    PSOD_LAUNCH_CHECK();
+    k_centre_group<<<resident((const void *)k_centre_group), 256, 0, st>>>(
+        c, cat->cell_start, (const float4 *)cat->rec, plan, large, tasks, ntot,
+        items, nitems, hbest, cub, ires, cstat, wq + 2);
+    PSOD_LAUNCH_CHECK();
+    k_centre_reduce<<<grid_for(nlarge, 256), 256, 0, st>>>(plan, large, nlarge,
+                                                           tres, ires);
     PSOD_LAUNCH_CHECK();
     PSOD_HIP(hipEventRecord(ctx->ev[1], st));
     // read back with the growth's counters (the stream is synchronised
