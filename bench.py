@@ -270,9 +270,9 @@ def main():
             pair_tests=cs["pair_tests"], ms=round(cs["centre_ms"], 3),
             pair_tests_per_s=(round(cs["pair_tests"] / (cs["centre_ms"] * 1e-3), 1)
                               if cs["centre_ms"] > 0 else None),
-            tasks=cs["centre_tasks_all"], evaluated=cs["centre_tasks"],
-            skipped_by_bound=cs["centre_skipped"],
-            abandoned=cs["centre_aborted"]),
+            tasks=cs["centre_tasks_all"], units_evaluated=cs["centre_tasks"],
+            units_skipped_by_bound=cs["centre_skipped"],
+            groups_abandoned=cs["centre_aborted"]),
         "sod_paths": dict(wave=cs["wave"], workgroup=cs["workgroup"],
                           rerun=cs["retried"]),
         "plain_fp64": plain,

@@ -242,8 +242,11 @@ class Context:
     def sod_stats(self):
         """Execution-path counts of the last sod_fof call: halos grown by
         one wave, by one workgroup, re-run with larger collection slices,
-        the keys per slice of that re-run, and the centre search's tasks
-        (evaluated, skipped by the bound, in all) and pair tests."""
+        the keys per slice of that re-run, and the centre search of the
+        large halos: its tasks (64 candidates of one cell) in all, units
+        evaluated exactly (pilot tasks and groups of 8 candidates), units
+        skipped by their bound, groups abandoned part-way, pair tests and
+        the search's time."""
         a = (C.c_int64 * 10)()
         _lib.check(_L.psod_sod_stats(self.h, a, 10))
         return dict(wave=a[0], workgroup=a[1], retried=a[2], retry_slice=a[3],

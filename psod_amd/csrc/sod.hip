@@ -1325,17 +1325,19 @@ __global__ void k_res_init(uint64_t *__restrict__ r, int64_t n) {
 
 // Each halo's centre: the best (S, key) over its pilot tasks and the groups
 // of its other tasks, key = (candidate cell rank << 32) | particle index.
+// One wave per halo, its lanes over the tasks.
 __global__ void __launch_bounds__(256)
 k_centre_reduce(HaloPlan *__restrict__ plan, const int32_t *__restrict__ large,
                 int32_t nlarge, const uint64_t *__restrict__ taskres,
                 const uint64_t *__restrict__ itemres) {
-  const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nlarge) return;
+  const int l = threadIdx.x & 63;
+  const int32_t e = blockIdx.x * 4 + (int32_t)(threadIdx.x >> 6);
+  if (e >= nlarge) return;  // uniform per wave
   const int32_t h = large[e];
-  HaloPlan pl = plan[h];
+  const HaloPlan pl = plan[h];
   if (pl.centre >= 0 || pl.ntask == 0) return;
   uint64_t ba = 0, bk = ~0ull;
-  for (int32_t t = pl.task0; t < pl.task0 + pl.ntask; t++) {
+  for (int32_t t = pl.task0 + l; t < pl.task0 + pl.ntask; t += 64) {
     const uint64_t a = taskres[2 * (int64_t)t], k = taskres[2 * (int64_t)t + 1];
     if (better(a, k, ba, bk)) { ba = a; bk = k; }
     for (int g = 0; g < 8; g++) {
@@ -1344,7 +1346,8 @@ k_centre_reduce(HaloPlan *__restrict__ plan, const int32_t *__restrict__ large,
       if (better(a2, k2, ba, bk)) { ba = a2; bk = k2; }
     }
   }
-  plan[h].centre = key_idx(bk);
+  wave_argmax(ba, bk);
+  if (l == 0) plan[h].centre = key_idx(bk);
 }
 
 // ---------------------------------------------------------------------------
@@ -2245,8 +2248,8 @@ extern "C" int psod_sod_fof(psod_ctx *ctx, const psod_sod_params *p,
         c, cat->cell_start, (const float4 *)cat->rec, plan, large, tasks, ntot,
         items, nitems, hbest, cub, ires, cstat, wq + 2);
     PSOD_LAUNCH_CHECK();
-    k_centre_reduce<<<grid_for(nlarge, 256), 256, 0, st>>>(plan, large, nlarge,
-                                                           tres, ires);
+    k_centre_reduce<<<grid_for(nlarge, 4), 256, 0, st>>>(plan, large, nlarge,
+                                                         tres, ires);
     PSOD_LAUNCH_CHECK();
     PSOD_HIP(hipEventRecord(ctx->ev[1], st));
     // read back with the growth's counters (the stream is synchronised
