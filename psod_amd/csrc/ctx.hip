@@ -39,7 +39,10 @@ extern "C" int psod_create(int device, psod_ctx **out) {
                           hipGetErrorString(es));
   }
   if (hipEventCreate(&c->ev[0]) != hipSuccess ||
-      hipEventCreate(&c->ev[1]) != hipSuccess) {
+      hipEventCreate(&c->ev[1]) != hipSuccess ||
+      hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess) {
     hipStreamDestroy(c->stream);
     free(c);
     return psod_set_error(PSOD_E_DEVICE, "hipEventCreate failed");
@@ -57,10 +60,14 @@ extern "C" int psod_destroy(psod_ctx *c) {
   if (!c) return PSOD_OK;
   hipSetDevice(c->device);
   hipStreamSynchronize(c->stream);
+  hipStreamSynchronize(c->side);
   for (int s = 0; s < S_NSLOT; s++)
     if (c->slot[s]) hipFree(c->slot[s]);
   hipEventDestroy(c->ev[0]);
   hipEventDestroy(c->ev[1]);
+  hipEventDestroy(c->fork);
+  hipEventDestroy(c->join);
+  hipStreamDestroy(c->side);
   hipStreamDestroy(c->stream);
   free(c);
   return PSOD_OK;
@@ -69,6 +76,7 @@ extern "C" int psod_destroy(psod_ctx *c) {
 extern "C" int psod_synchronize(psod_ctx *c) {
   if (!c) return psod_set_error(PSOD_E_ARG, "null context");
   PSOD_HIP(hipStreamSynchronize(c->stream));
+  PSOD_HIP(hipStreamSynchronize(c->side));
   return PSOD_OK;
 }
 

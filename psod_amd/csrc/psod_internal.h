@@ -78,6 +78,10 @@ struct psod_ctx {
   int num_cus;
   int64_t sod_stat[10];  // see psod_sod_stats
   hipEvent_t ev[2];     // brackets the centre search (sod_stat[8])
+  // second stream (default priority) for work that overlaps the main one,
+  // and the events that fork to it and join from it
+  hipStream_t side;
+  hipEvent_t fork, join;
 };
 
 // Returns a device buffer of at least `bytes` in `slot` (contents not kept

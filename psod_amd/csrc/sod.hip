@@ -325,14 +325,20 @@ k_sod_small(SodCfg c, const int32_t *__restrict__ cell_start,
     __syncthreads();
     // ---- centre search -------------------------------------------------
     uint64_t bacc = 0, bkey = ~0ull;
-    for (int32_t ci = l; ci < ncand; ci += 64) {
-      const int j = S.a.cidx[ci];
+    // every lane takes part in each round (the neighbour loop is uniform:
+    // scalar bounds, broadcast LDS reads, four in flight)
+    const int32_t ncu = __builtin_amdgcn_readfirstlane(ncand);
+    const int32_t n5u = __builtin_amdgcn_readfirstlane(n5);
+    for (int32_t cb = 0; cb < ncu; cb += 64) {
+      const bool have = cb + l < ncu;
+      const int j = have ? S.a.cidx[cb + l] : 0;
       const float4 p = S.a.nb[j];
       uint64_t acc = 0;
       // u32 partial sums of at most 256 terms of at most 2^24
-      for (int32_t t0 = 0; t0 < n5; t0 += 256) {
-        const int32_t t1 = min(n5, t0 + 256);
+      for (int32_t t0 = 0; t0 < n5u; t0 += 256) {
+        const int32_t t1 = min(n5u, t0 + 256);
         uint32_t part = 0;
+#pragma unroll 4
         for (int32_t t = t0; t < t1; t++) {
           const float4 q = S.a.nb[t];
           const float dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
@@ -342,9 +348,11 @@ k_sod_small(SodCfg c, const int32_t *__restrict__ cell_start,
         }
         acc += part;
       }
-      const uint64_t key = ((uint64_t)(uint32_t)S.a.rank[j] << 32) |
-                           (uint32_t)__float_as_int(p.w);
-      if (better(acc, key, bacc, bkey)) { bacc = acc; bkey = key; }
+      if (have) {
+        const uint64_t key = ((uint64_t)(uint32_t)S.a.rank[j] << 32) |
+                             (uint32_t)__float_as_int(p.w);
+        if (better(acc, key, bacc, bkey)) { bacc = acc; bkey = key; }
+      }
     }
     wave_argmax(bacc, bkey);
     const int32_t centre = key_idx(bkey);
@@ -2122,65 +2130,16 @@ __global__ void k_large_keys(const HaloPlan *__restrict__ plan,
   vals[e] = (uint32_t)h;
 }
 
-extern "C" int psod_sod_fof(psod_ctx *ctx, const psod_sod_params *p,
-                            const psod_catalogue *cat, int64_t nhalo,
-                            const int32_t *d_seed, const int32_t *d_nmem,
-                            const psod_halo_out *out) {
-  PSOD_TRY(psod_use_device(ctx));
-  PSOD_TRY(check_sod_args(p, cat, nhalo, d_seed, out));
-  if (nhalo <= 0) return PSOD_OK;
-  if (!out->profile)
-    return psod_set_error(PSOD_E_ARG, "sod: profile output is required");
-  const SodCfg c = make_cfg(p, cat->npart);
+// Centre search of the first nlarge halos of `large` (positions e): tasks
+// of 64 candidates, their bounds, pilots, groups, and each halo's centre in
+// plan[].centre.  ntask, task0 and ntot are scratch of nlarge + 16 entries
+// (ntot: 16); hbest holds 2 * nlarge + 4 entries.
+static int centre_search(psod_ctx *ctx, const SodCfg &c,
+                         const psod_catalogue *cat, HaloPlan *plan,
+                         const int32_t *large, int32_t nlarge, int32_t *ntask,
+                         int32_t *task0, int32_t *ntot,
+                         unsigned long long *hbest) {
   hipStream_t st = ctx->stream;
-  // plan, lists and counters
-  HaloPlan *plan = (HaloPlan *)psod_scratch(ctx, S_PLAN,
-                                            (size_t)nhalo * sizeof(HaloPlan));
-  int32_t *lists = (int32_t *)psod_scratch(ctx, S_TASK,
-                                           (size_t)(2 * nhalo + 64) * 4);
-  if (!plan || !lists) return PSOD_E_NOMEM;
-  int32_t *small = lists, *large = lists + nhalo, *counters = lists + 2 * nhalo;
-  PSOD_TRY(psod_fill_i32(ctx, counters, 40, 0));
-  k_plan<<<grid_for(nhalo, 256), 256, 0, st>>>(
-      c, nhalo, d_seed, d_nmem, cat->pos, cat->cell_start, plan, small, large,
-      counters, *out);
-  PSOD_LAUNCH_CHECK();
-  // small halos, persistent: a few workgroups per CU
-  k_sod_small<<<(unsigned)ctx->num_cus * 8, 64, 0, st>>>(
-      c, cat->cell_start, (const float4 *)cat->rec, cat->pos, cat->vel,
-      cat->ids, plan, small, large, counters, *out);
-  PSOD_LAUNCH_CHECK();
-  int32_t hc[4];
-  PSOD_HIP(hipMemcpyAsync(hc, counters, 16, hipMemcpyDeviceToHost, st));
-  PSOD_HIP(hipStreamSynchronize(st));
-  const int32_t nlarge = hc[1];
-  ctx->sod_stat[0] = nhalo - nlarge;
-  ctx->sod_stat[1] = nlarge;
-  for (int i = 2; i < 10; i++) ctx->sod_stat[i] = 0;
-  if (nlarge == 0) return PSOD_OK;
-  // largest halos first, so that the persistent growth workgroups do not
-  // finish on a giant halo picked up last
-  {
-    uint64_t *keys = (uint64_t *)psod_scratch(ctx, S_KEYS, (size_t)nlarge * 8);
-    uint32_t *vals = (uint32_t *)psod_scratch(ctx, S_VALS, (size_t)nlarge * 4);
-    if (!keys || !vals) return PSOD_E_NOMEM;
-    k_large_keys<<<grid_for(nlarge, 256), 256, 0, st>>>(plan, large, nlarge,
-                                                        keys, vals);
-    PSOD_LAUNCH_CHECK();
-    PSOD_TRY(psod_radix_sort_u64(ctx, nlarge, keys, vals, 32));
-    PSOD_HIP(hipMemcpyAsync(large, vals, (size_t)nlarge * 4,
-                            hipMemcpyDeviceToDevice, st));
-  }
-  // centre-search tasks of the large halos
-  int32_t *ntask = (int32_t *)psod_scratch(
-      ctx, S_MISC, (size_t)(5 * nlarge + 128) * 4 + (size_t)nlarge * 16 + 512);
-  if (!ntask) return PSOD_E_NOMEM;
-  // best S found so far per large halo (branch and bound of the search)
-  unsigned long long *hbest = (unsigned long long *)(
-      (((uintptr_t)(ntask + 5 * nlarge + 128)) + 255) & ~(uintptr_t)255);
-  int32_t *task0 = ntask + nlarge + 16;
-  int32_t *ntot = task0 + nlarge + 16;
-  int32_t *retry = ntot + 16;  // positions of halos that need larger slices
   k_task_count<<<grid_for(nlarge, 256), 256, 0, st>>>(c, cat->cell_start,
                                                        plan, large, nlarge,
                                                        ntask);
@@ -2260,6 +2219,87 @@ extern "C" int psod_sod_fof(psod_ctx *ctx, const psod_sod_params *p,
                             hipMemcpyDeviceToHost, st));
     ctx->sod_stat[7] = nt;
   }
+  return PSOD_OK;
+}
+
+extern "C" int psod_sod_fof(psod_ctx *ctx, const psod_sod_params *p,
+                            const psod_catalogue *cat, int64_t nhalo,
+                            const int32_t *d_seed, const int32_t *d_nmem,
+                            const psod_halo_out *out) {
+  PSOD_TRY(psod_use_device(ctx));
+  PSOD_TRY(check_sod_args(p, cat, nhalo, d_seed, out));
+  if (nhalo <= 0) return PSOD_OK;
+  if (!out->profile)
+    return psod_set_error(PSOD_E_ARG, "sod: profile output is required");
+  const SodCfg c = make_cfg(p, cat->npart);
+  hipStream_t st = ctx->stream;
+  // plan, lists and counters
+  HaloPlan *plan = (HaloPlan *)psod_scratch(ctx, S_PLAN,
+                                            (size_t)nhalo * sizeof(HaloPlan));
+  int32_t *lists = (int32_t *)psod_scratch(ctx, S_TASK,
+                                           (size_t)(2 * nhalo + 64) * 4);
+  if (!plan || !lists) return PSOD_E_NOMEM;
+  int32_t *small = lists, *large = lists + nhalo, *counters = lists + 2 * nhalo;
+  PSOD_TRY(psod_fill_i32(ctx, counters, 40, 0));
+  k_plan<<<grid_for(nhalo, 256), 256, 0, st>>>(
+      c, nhalo, d_seed, d_nmem, cat->pos, cat->cell_start, plan, small, large,
+      counters, *out);
+  PSOD_LAUNCH_CHECK();
+  int32_t hc[4];
+  PSOD_HIP(hipMemcpyAsync(hc, counters, 16, hipMemcpyDeviceToHost, st));
+  PSOD_HIP(hipStreamSynchronize(st));
+  // halos of the large path known from the plan; the small path appends the
+  // ones it defers (their centre found) behind them
+  const int32_t nlarge1 = hc[1];
+  // small halos, persistent (a few workgroups per CU), on the side stream:
+  // they overlap the centre search of the large halos, which touches other
+  // halos' plans and outputs
+  PSOD_HIP(hipEventRecord(ctx->fork, st));
+  PSOD_HIP(hipStreamWaitEvent(ctx->side, ctx->fork, 0));
+  k_sod_small<<<(unsigned)ctx->num_cus * 8, 64, 0, ctx->side>>>(
+      c, cat->cell_start, (const float4 *)cat->rec, cat->pos, cat->vel,
+      cat->ids, plan, small, large, counters, *out);
+  PSOD_LAUNCH_CHECK();
+  PSOD_HIP(hipEventRecord(ctx->join, ctx->side));
+  for (int i = 0; i < 10; i++) ctx->sod_stat[i] = 0;
+  // largest halos first (centre-search tasks and growth), so that the
+  // persistent workgroups do not finish on a giant halo picked up last
+  auto size_order = [&](int32_t n) -> int {
+    uint64_t *keys = (uint64_t *)psod_scratch(ctx, S_KEYS, (size_t)n * 8);
+    uint32_t *vals = (uint32_t *)psod_scratch(ctx, S_VALS, (size_t)n * 4);
+    if (!keys || !vals) return PSOD_E_NOMEM;
+    k_large_keys<<<grid_for(n, 256), 256, 0, st>>>(plan, large, n, keys, vals);
+    PSOD_LAUNCH_CHECK();
+    PSOD_TRY(psod_radix_sort_u64(ctx, n, keys, vals, 32));
+    PSOD_HIP(hipMemcpyAsync(large, vals, (size_t)n * 4,
+                            hipMemcpyDeviceToDevice, st));
+    return PSOD_OK;
+  };
+  // the S_MISC layout below is sized for every halo: the growth's retry
+  // list holds up to all of the large halos
+  const int32_t ncap = (int32_t)nhalo;
+  int32_t *ntask = (int32_t *)psod_scratch(
+      ctx, S_MISC, (size_t)(5 * ncap + 128) * 4 + (size_t)ncap * 16 + 512);
+  if (!ntask) return PSOD_E_NOMEM;
+  // best S found so far per large halo (branch and bound of the search)
+  unsigned long long *hbest = (unsigned long long *)(
+      (((uintptr_t)(ntask + 5 * ncap + 128)) + 255) & ~(uintptr_t)255);
+  int32_t *task0 = ntask + ncap + 16;
+  int32_t *ntot = task0 + ncap + 16;
+  int32_t *retry = ntot + 16;  // positions of halos that need larger slices
+  if (nlarge1 > 0) {
+    PSOD_TRY(size_order(nlarge1));
+    PSOD_TRY(centre_search(ctx, c, cat, plan, large, nlarge1, ntask, task0,
+                           ntot, hbest));
+  }
+  PSOD_HIP(hipStreamWaitEvent(st, ctx->join, 0));
+  PSOD_HIP(hipMemcpyAsync(hc, counters, 16, hipMemcpyDeviceToHost, st));
+  PSOD_HIP(hipStreamSynchronize(st));
+  const int32_t nlarge = hc[1];
+  ctx->sod_stat[0] = nhalo - nlarge;
+  ctx->sod_stat[1] = nlarge;
+  if (nlarge == 0) return PSOD_OK;
+  if (nlarge > nlarge1) PSOD_TRY(size_order(nlarge));
   // one 512-thread workgroup per CU (the kernel's registers allow 8 waves)
   const unsigned ngrow = (unsigned)min(nlarge, ctx->num_cus);
   int32_t *hist = (int32_t *)psod_scratch(ctx, S_BINS,
