@@ -8,6 +8,8 @@
 // subtraction from x*nx, itself exact for a power-of-two nx).
 #include "psod_internal.h"
 
+#include <algorithm>
+
 __global__ void k_fill_i32(int32_t *__restrict__ p, int64_t n, int32_t v) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -236,8 +238,18 @@ __global__ void k_cs_split(const int32_t *__restrict__ O, int dbits,
   bs2[B] = ntb[b] > 0 ? O[(int64_t)tb[b] * nd + (int64_t)d * ntb[b]] : bs[b];
 }
 
-// one workgroup per block: counting sort of its records by local cell
-// (LDS), final records and the block's 512 cell offsets
+// One workgroup per block: counting sort of its records by local cell, the
+// final records and the block's 512 cell offsets.  A block of at most BS_CAP
+// records is read once into registers, placed in LDS in cell order and
+// written out as one contiguous run; a larger one is read twice (count, then
+// place) and written record by record.
+constexpr int BS_CAP = 2048;
+constexpr int BS_U = BS_CAP / 256;
+
+__device__ inline int32_t local_cell(const float4 &r) {
+  return ((int32_t)r.z * 8 + (int32_t)r.y) * 8 + (int32_t)r.x;
+}
+
 __global__ void __launch_bounds__(256)
 k_block_sort(const int32_t *__restrict__ bstart, int64_t nblk,
              const float4 *__restrict__ stage,
@@ -245,14 +257,28 @@ k_block_sort(const int32_t *__restrict__ bstart, int64_t nblk,
              int32_t *__restrict__ cell_start, float4 *__restrict__ rec,
              int32_t *__restrict__ perm, int32_t *__restrict__ lab_sorted) {
   __shared__ int32_t cnt[512];
+  __shared__ float4 sr[BS_CAP];
+  __shared__ int32_t sl[BS_CAP];
   for (int64_t b = blockIdx.x; b < nblk; b += gridDim.x) {
     const int32_t a = bstart[b], n = bstart[b + 1] - a;
     for (int i = threadIdx.x; i < 512; i += 256) cnt[i] = 0;
     __syncthreads();
-    for (int32_t t = threadIdx.x; t < n; t += 256) {
-      const float4 r = stage[a + t];
-      const int32_t l = ((int32_t)r.z * 8 + (int32_t)r.y) * 8 + (int32_t)r.x;
-      atomicAdd(&cnt[l], 1);
+    const bool staged = n <= BS_CAP;
+    float4 rr[BS_U];
+    int32_t ll[BS_U], dd[BS_U];
+    if (staged) {
+#pragma unroll
+      for (int k = 0; k < BS_U; k++) {
+        const int32_t t = threadIdx.x + 256 * k;
+        if (t < n) {
+          rr[k] = stage[a + t];
+          ll[k] = stage_lab ? stage_lab[a + t] : 0;
+          dd[k] = atomicAdd(&cnt[local_cell(rr[k])], 1);
+        }
+      }
+    } else {
+      for (int32_t t = threadIdx.x; t < n; t += 256)
+        atomicAdd(&cnt[local_cell(stage[a + t])], 1);
     }
     __syncthreads();
     if (threadIdx.x < 64) {  // exclusive scan of 512 counters by one wave
@@ -267,13 +293,31 @@ k_block_sort(const int32_t *__restrict__ bstart, int64_t nblk,
       }
     }
     __syncthreads();
-    for (int32_t t = threadIdx.x; t < n; t += 256) {
-      const float4 r = stage[a + t];
-      const int32_t l = ((int32_t)r.z * 8 + (int32_t)r.y) * 8 + (int32_t)r.x;
-      const int32_t d = a + atomicAdd(&cnt[l], 1);
-      rec[d] = r;
-      if (perm) perm[d] = __float_as_int(r.w);
-      if (lab_sorted) lab_sorted[d] = stage_lab[a + t];
+    if (staged) {
+#pragma unroll
+      for (int k = 0; k < BS_U; k++) {
+        const int32_t t = threadIdx.x + 256 * k;
+        if (t < n) {
+          const int32_t p = cnt[local_cell(rr[k])] + dd[k];
+          sr[p] = rr[k];
+          sl[p] = ll[k];
+        }
+      }
+      __syncthreads();
+      for (int32_t t = threadIdx.x; t < n; t += 256) {
+        const float4 r = sr[t];
+        rec[a + t] = r;
+        if (perm) perm[a + t] = __float_as_int(r.w);
+        if (lab_sorted) lab_sorted[a + t] = sl[t];
+      }
+    } else {
+      for (int32_t t = threadIdx.x; t < n; t += 256) {
+        const float4 r = stage[a + t];
+        const int32_t d = a + atomicAdd(&cnt[local_cell(r)], 1);
+        rec[d] = r;
+        if (perm) perm[d] = __float_as_int(r.w);
+        if (lab_sorted) lab_sorted[d] = stage_lab[a + t];
+      }
     }
     __syncthreads();
   }
@@ -306,15 +350,34 @@ __device__ inline uint32_t morton9(const float4 &r) {
   return spread3(qx) | (spread3(qy) << 1) | (spread3(qz) << 2);
 }
 
+// dense cells that fit the registers of one workgroup go to list[0..),
+// larger ones to big[0..); cnt[0] and cnt[2] count them (one atomic per
+// wave and list)
 __global__ void k_dense_cells(const int32_t *__restrict__ cs, int64_t ncell,
                               int32_t *__restrict__ list,
+                              int32_t *__restrict__ big,
                               int32_t *__restrict__ cnt, int32_t cap) {
-  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < ncell;
-       c += (int64_t)gridDim.x * blockDim.x)
-    if (cs[c + 1] - cs[c] >= DENSE) {
-      const int32_t k = atomicAdd(cnt, 1);
-      if (k < cap) list[k] = (int32_t)c;
+  const int l = threadIdx.x & 63;
+  const uint64_t below = l == 0 ? 0ull : (~0ull >> (64 - l));
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t c0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63);
+       c0 < ncell; c0 += stride) {
+    const int64_t c = c0 + l;
+    const int32_t n = c < ncell ? cs[c + 1] - cs[c] : 0;
+    const bool dn = n >= DENSE, reg = n <= SUB_REG;
+    const uint64_t m0 = __ballot(dn && reg), m1 = __ballot(dn && !reg);
+    int32_t b0 = 0, b1 = 0;
+    if (l == 0) {
+      if (m0) b0 = atomicAdd(&cnt[0], __popcll(m0));
+      if (m1) b1 = atomicAdd(&cnt[2], __popcll(m1));
     }
+    b0 = __shfl(b0, 0, 64);
+    b1 = __shfl(b1, 0, 64);
+    if (dn) {
+      const int32_t k = reg ? b0 + __popcll(m0 & below) : b1 + __popcll(m1 & below);
+      if (k < cap) (reg ? list : big)[k] = (int32_t)c;
+    }
+  }
 }
 
 __global__ void __launch_bounds__(SUB_T)
@@ -327,9 +390,9 @@ k_subsort(const int32_t *__restrict__ cs, const int32_t *__restrict__ list,
   __shared__ int32_t hist[512];
   __shared__ int32_t item;
   const int32_t nl = min(*nlist, cap);
-  float4 *tr = trec + (int64_t)blockIdx.x * SUB_TMP;
-  int32_t *tp = tperm + (int64_t)blockIdx.x * SUB_TMP;
-  int32_t *tl = tlab + (int64_t)blockIdx.x * SUB_TMP;
+  float4 *tr = trec ? trec + (int64_t)blockIdx.x * SUB_TMP : nullptr;
+  int32_t *tp = tperm ? tperm + (int64_t)blockIdx.x * SUB_TMP : nullptr;
+  int32_t *tl = tlab ? tlab + (int64_t)blockIdx.x * SUB_TMP : nullptr;
   for (;;) {
     if (threadIdx.x == 0) item = atomicAdd(queue, 1);
     for (int i = threadIdx.x; i < 512; i += SUB_T) hist[i] = 0;
@@ -338,7 +401,8 @@ k_subsort(const int32_t *__restrict__ cs, const int32_t *__restrict__ list,
     if (it >= nl) break;
     const int32_t c = list[it];
     const int32_t a = cs[c], n = cs[c + 1] - a;
-    if (n > SUB_TMP) {  // would not fit the slice: keep the scatter order
+    if (n > SUB_TMP || (n > SUB_REG && !trec)) {
+      // would not fit the slice: keep the scatter order
       __syncthreads();
       continue;
     }
@@ -495,22 +559,29 @@ extern "C" int psod_cell_sort(psod_ctx *ctx, int64_t npart, const float *d_pos,
   PSOD_LAUNCH_CHECK();
   if (npart > 0) {
     // Morton order of the 1/8-cell sub-cells inside dense cells
+    // (at most npart / DENSE cells are dense)
+    const int32_t cap = (int32_t)std::min<int64_t>(ncell, npart / DENSE + 32);
     int32_t *list = (int32_t *)psod_scratch(ctx, S_HIST,
-                                            (size_t)(ncell / DENSE + 64) * 4);
+                                            (size_t)(2 * (int64_t)cap + 64) * 4);
     int32_t *ctl = (int32_t *)psod_scratch(ctx, S_MISC, 64);
     if (!list || !ctl) return PSOD_E_NOMEM;
-    PSOD_TRY(psod_fill_i32(ctx, ctl, 2, 0));
+    int32_t *big = list + cap + 32;
+    PSOD_TRY(psod_fill_i32(ctx, ctl, 4, 0));
     k_dense_cells<<<grid_for(ncell, 256, (int64_t)ctx->num_cus * 64), 256, 0,
-                    st>>>(d_cell_start, ncell, list, ctl,
-                          (int32_t)(ncell / DENSE + 32));
+                    st>>>(d_cell_start, ncell, list, big, ctl, cap);
+    PSOD_LAUNCH_CHECK();
+    // cells that fit the registers: many workgroups, no staging slices;
+    // larger cells: one workgroup per CU, each with a staging slice
+    k_subsort<<<(unsigned)ctx->num_cus * 8, SUB_T, 0, st>>>(
+        d_cell_start, list, ctl, cap, ctl + 1, (float4 *)d_rec, d_perm,
+        d_lab_sorted, nullptr, nullptr, nullptr);
     PSOD_LAUNCH_CHECK();
     const unsigned nsub = (unsigned)ctx->num_cus;
     char *tmp = (char *)psod_scratch(ctx, S_KEYS,
                                      (size_t)nsub * SUB_TMP * 24);
     if (!tmp) return PSOD_E_NOMEM;
     k_subsort<<<nsub, SUB_T, 0, st>>>(
-        d_cell_start, list, ctl, (int32_t)(ncell / DENSE + 32), ctl + 1,
-        (float4 *)d_rec, d_perm,
+        d_cell_start, big, ctl + 2, cap, ctl + 3, (float4 *)d_rec, d_perm,
         d_lab_sorted, (float4 *)tmp,
         (int32_t *)(tmp + (size_t)nsub * SUB_TMP * 16),
         (int32_t *)(tmp + (size_t)nsub * SUB_TMP * 20));
