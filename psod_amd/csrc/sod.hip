@@ -1372,6 +1372,9 @@ constexpr int MAXRANGE = R2 + SEGCAP;
 // radial histogram of one workgroup: HIST_MIN..HIST_MAX bins (powers of
 // two, a multiple of GL_T) and the total
 constexpr int32_t HIST_MIN = 4096, HIST_MAX = 1 << 20;
+// the first HIST_LDS bins (the dense inner shells) are counted in LDS
+constexpr int32_t HIST_LDS = 16384;
+constexpr int32_t SCAN_ROUND = 4096;  // bins per round of the scan
 constexpr int64_t HIST_SLICE = HIST_MAX + 64;
 
 struct GrowShared {
@@ -1382,7 +1385,7 @@ struct GrowShared {
   int64_t red_l[GL_T / 64];
   double red_d[GL_T / 64];
   double prad[257];
-  int32_t pcnt[257];
+  int32_t pcnt[GL_T / 64][257];  // per wave
   int32_t item;
   int32_t fill[MAXRANGE];
   int64_t off[MAXRANGE];
@@ -1670,6 +1673,7 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
   // is appended to `retry` (when given) with the slice size it needs
   // folded into *needmax, for a second launch with larger slices.
   __shared__ GrowShared G;
+  __shared__ int32_t lh[HIST_LDS];  // inner bins; staging of the scan
   const int32_t nitem = counters[1];
   // this workgroup's slices of the collection pool and of the radial
   // histogram, reused for every halo
@@ -1707,7 +1711,9 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
       int32_t nb = HIST_MIN;
       while ((int64_t)nb * 4 < ncube && nb < HIST_MAX) nb <<= 1;
       const float scale = (float)nb / ((float)W * (float)W * (float)W);
-      for (int32_t b = threadIdx.x; b <= nb; b += GL_T) hist[b] = 0;
+      const int32_t nl = min(nb, HIST_LDS);
+      for (int32_t b = threadIdx.x; b < nl; b += GL_T) lh[b] = 0;
+      for (int32_t b = nl + threadIdx.x; b <= nb; b += GL_T) hist[b] = 0;
       __threadfence_block();
       __syncthreads();
       // ---- pass 1: histogram -------------------------------------------
@@ -1716,32 +1722,55 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
                 [&](float d2, int32_t, const float4 &, int, int, int,
                     const int32_t *) {
                   ng++;
-                  if (d2 < W2) atomicAdd(&hist[rbin(d2, scale, nb)], 1);
+                  if (d2 < W2) {
+                    const int32_t b = rbin(d2, scale, nb);
+                    if (b < HIST_LDS) atomicAdd(&lh[b], 1);
+                    else atomicAdd(&hist[b], 1);
+                  }
                 });
       ng = block_sum_i64(ng, G);
+      for (int32_t b = threadIdx.x; b < nl; b += GL_T) hist[b] = lh[b];
       __threadfence();
       __syncthreads();
-      // exclusive scan in place: hist[b] = #(bin < b), hist[nb] = total;
-      // thread t owns the nb / GL_T consecutive bins from t * nb / GL_T
+      // exclusive scan in place: hist[b] = #(bin < b), hist[nb] = total.
+      // Rounds of SCAN_ROUND bins, read coalesced into LDS; thread t scans
+      // the SCAN_ROUND / GL_T consecutive bins from t * SCAN_ROUND / GL_T.
       {
-        const int32_t per = nb / GL_T;
-        int32_t *hp = hist + threadIdx.x * per;
-        int32_t sum = 0;
-        for (int32_t i = 0; i < per; i++) sum += ldcg(&hp[i]);
+        constexpr int PER = SCAN_ROUND / GL_T;
         const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-        int32_t tot;
-        const int32_t ex = wave_excl(sum, &tot);
-        if (l == 0) G.red_i[w] = tot;
-        __syncthreads();
-        int32_t run = ex;
-        for (int i = 0; i < w; i++) run += G.red_i[i];
-        for (int32_t i = 0; i < per; i++) {
-          const int32_t v = ldcg(&hp[i]);
-          hp[i] = run;
-          run += v;
+        int32_t carry = 0;
+        for (int32_t base = 0; base < nb; base += SCAN_ROUND) {
+          for (int i = threadIdx.x; i < SCAN_ROUND; i += GL_T)
+            lh[i] = ldcg(&hist[base + i]);
+          __syncthreads();
+          int32_t v[PER], sum = 0;
+#pragma unroll
+          for (int i = 0; i < PER; i++) {
+            v[i] = lh[threadIdx.x * PER + i];
+            sum += v[i];
+          }
+          int32_t tot;
+          const int32_t ex = wave_excl(sum, &tot);
+          if (l == 0) G.red_i[w] = tot;
+          __syncthreads();
+          int32_t run = carry + ex, all = 0;
+          for (int i = 0; i < GL_T / 64; i++) {
+            if (i < w) run += G.red_i[i];
+            all += G.red_i[i];
+          }
+#pragma unroll
+          for (int i = 0; i < PER; i++) {
+            lh[threadIdx.x * PER + i] = run;
+            run += v[i];
+          }
+          __syncthreads();
+          for (int i = threadIdx.x; i < SCAN_ROUND; i += GL_T)
+            hist[base + i] = lh[i];
+          carry += all;
+          __syncthreads();
         }
-        if (threadIdx.x == GL_T - 1) hist[nb] = run;
-        __threadfence_block();
+        if (threadIdx.x == 0) hist[nb] = carry;
+        __threadfence();
         __syncthreads();
       }
       const int32_t *C = hist;
@@ -1989,7 +2018,7 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
       // profile radii
       for (int j = threadIdx.x; j < c.nbins; j += GL_T) {
         G.prad[j] = prof_radius(c, R0, j);
-        G.pcnt[j] = 0;
+        for (int w = 0; w < GL_T / 64; w++) G.pcnt[w][j] = 0;
       }
       __syncthreads();
       if (G.need > pool_cap) { failed = true; break; }
@@ -2009,7 +2038,7 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
                       const int mid = (lo + hi) >> 1;
                       if (rr < G.prad[mid]) hi = mid; else lo = mid + 1;
                     }
-                    atomicAdd(&G.pcnt[lo], 1);
+                    atomicAdd(&G.pcnt[threadIdx.x >> 6][lo], 1);
                   }
                   if (fabs(rr - R0) <= c.tol_abs) nbd++;
                   if (d2key(d2, idx) <= keyR) {
@@ -2046,7 +2075,7 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
       if (threadIdx.x == 0) {
         int32_t s = 0;
         for (int j = 0; j < c.nbins; j++) {
-          s += G.pcnt[j];
+          for (int w = 0; w < GL_T / 64; w++) s += G.pcnt[w][j];
           out.profile[h * c.nbins + j] = s;
         }
       }
