@@ -2146,16 +2146,39 @@ __global__ void k_retry_reset(int32_t *__restrict__ c, int32_t nitem,
   if (blockIdx.x == 0 && threadIdx.x == 0) *need = 0ull;
 }
 
+// Sort keys of the large halos, most costly first.  For the centre search
+// the cost is the particles near the seed (members plus those of the 5^3
+// cells).  For the growth it is the particles of the cube of the planned
+// reach, summed over the 8^3 blocks that the cube overlaps: a small halo
+// beside a giant one walks the giant's particles too.
 __global__ void k_large_keys(const HaloPlan *__restrict__ plan,
                              const int32_t *__restrict__ large, int32_t n,
+                             const int32_t *__restrict__ cell_start,
+                             int32_t nx, int growth,
                              uint64_t *__restrict__ keys,
                              uint32_t *__restrict__ vals) {
   const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   const int32_t h = large[e];
   const HaloPlan pl = plan[h];
-  const int64_t cost = (int64_t)max(pl.nmem, 0) + pl.n5;
-  keys[e] = (uint64_t)(0x7FFFFFFFll - min(cost, (int64_t)0x7FFFFFFF));
+  int64_t cost = (int64_t)max(pl.nmem, 0) + pl.n5;
+  if (growth) {
+    const int32_t nb = nx >> 3, W = max(pl.w0, 2);
+    int32_t lo[3], hi[3];
+    for (int d = 0; d < 3; d++) {
+      lo[d] = floor8(pl.cs[d] - W);
+      hi[d] = min(floor8(pl.cs[d] + W), lo[d] + nb - 1);  // at most nx cells
+    }
+    cost = 0;
+    for (int32_t bz = lo[2]; bz <= hi[2]; bz++)
+      for (int32_t by = lo[1]; by <= hi[1]; by++)
+        for (int32_t bx = lo[0]; bx <= hi[0]; bx++) {
+          const int64_t b = ((int64_t)wrapi(bz, nb) * nb + wrapi(by, nb)) * nb +
+                            wrapi(bx, nb);
+          cost += cell_start[(b + 1) * BS3] - cell_start[b * BS3];
+        }
+  }
+  keys[e] = (uint64_t)(0x7FFFFFFFFFFFll - min(cost, (int64_t)0x7FFFFFFFFFFFll));
   vals[e] = (uint32_t)h;
 }
 
@@ -2293,13 +2316,14 @@ extern "C" int psod_sod_fof(psod_ctx *ctx, const psod_sod_params *p,
   for (int i = 0; i < 10; i++) ctx->sod_stat[i] = 0;
   // largest halos first (centre-search tasks and growth), so that the
   // persistent workgroups do not finish on a giant halo picked up last
-  auto size_order = [&](int32_t n) -> int {
+  auto size_order = [&](int32_t n, int growth) -> int {
     uint64_t *keys = (uint64_t *)psod_scratch(ctx, S_KEYS, (size_t)n * 8);
     uint32_t *vals = (uint32_t *)psod_scratch(ctx, S_VALS, (size_t)n * 4);
     if (!keys || !vals) return PSOD_E_NOMEM;
-    k_large_keys<<<grid_for(n, 256), 256, 0, st>>>(plan, large, n, keys, vals);
+    k_large_keys<<<grid_for(n, 256), 256, 0, st>>>(
+        plan, large, n, cat->cell_start, c.nx, growth, keys, vals);
     PSOD_LAUNCH_CHECK();
-    PSOD_TRY(psod_radix_sort_u64(ctx, n, keys, vals, 32));
+    PSOD_TRY(psod_radix_sort_u64(ctx, n, keys, vals, 48));
     PSOD_HIP(hipMemcpyAsync(large, vals, (size_t)n * 4,
                             hipMemcpyDeviceToDevice, st));
     return PSOD_OK;
@@ -2317,7 +2341,7 @@ extern "C" int psod_sod_fof(psod_ctx *ctx, const psod_sod_params *p,
   int32_t *ntot = task0 + ncap + 16;
   int32_t *retry = ntot + 16;  // positions of halos that need larger slices
   if (nlarge1 > 0) {
-    PSOD_TRY(size_order(nlarge1));
+    PSOD_TRY(size_order(nlarge1, 0));
     PSOD_TRY(centre_search(ctx, c, cat, plan, large, nlarge1, ntask, task0,
                            ntot, hbest));
   }
@@ -2328,7 +2352,7 @@ extern "C" int psod_sod_fof(psod_ctx *ctx, const psod_sod_params *p,
   ctx->sod_stat[0] = nhalo - nlarge;
   ctx->sod_stat[1] = nlarge;
   if (nlarge == 0) return PSOD_OK;
-  if (nlarge > nlarge1) PSOD_TRY(size_order(nlarge));
+  PSOD_TRY(size_order(nlarge, 1));
   // one 512-thread workgroup per CU (the kernel's registers allow 8 waves)
   const unsigned ngrow = (unsigned)min(nlarge, ctx->num_cus);
   int32_t *hist = (int32_t *)psod_scratch(ctx, S_BINS,
