@@ -2186,11 +2186,14 @@ __global__ void k_large_keys(const HaloPlan *__restrict__ plan,
 // of 64 candidates, their bounds, pilots, groups, and each halo's centre in
 // plan[].centre.  ntask, task0 and ntot are scratch of nlarge + 16 entries
 // (ntot: 16); hbest holds 2 * nlarge + 4 entries.
+// after_setup() is called once the task count is known, before the
+// search's kernels are queued.
+template <typename F>
 static int centre_search(psod_ctx *ctx, const SodCfg &c,
                          const psod_catalogue *cat, HaloPlan *plan,
                          const int32_t *large, int32_t nlarge, int32_t *ntask,
                          int32_t *task0, int32_t *ntot,
-                         unsigned long long *hbest) {
+                         unsigned long long *hbest, F &&after_setup) {
   hipStream_t st = ctx->stream;
   k_task_count<<<grid_for(nlarge, 256), 256, 0, st>>>(c, cat->cell_start,
                                                        plan, large, nlarge,
@@ -2200,6 +2203,7 @@ static int centre_search(psod_ctx *ctx, const SodCfg &c,
   int32_t nt = 0;
   PSOD_HIP(hipMemcpyAsync(&nt, ntot, 4, hipMemcpyDeviceToHost, st));
   PSOD_HIP(hipStreamSynchronize(st));
+  PSOD_TRY(after_setup());
   if (nt > (1 << 27))
     return psod_set_error(PSOD_E_CAPACITY,
                           "%d centre-search tasks exceed 2^27", nt);
@@ -2303,16 +2307,20 @@ extern "C" int psod_sod_fof(psod_ctx *ctx, const psod_sod_params *p,
   // halos of the large path known from the plan; the small path appends the
   // ones it defers (their centre found) behind them
   const int32_t nlarge1 = hc[1];
-  // small halos, persistent (a few workgroups per CU), on the side stream:
+  // Small halos, persistent (a few workgroups per CU), on the side stream:
   // they overlap the centre search of the large halos, which touches other
-  // halos' plans and outputs
-  PSOD_HIP(hipEventRecord(ctx->fork, st));
-  PSOD_HIP(hipStreamWaitEvent(ctx->side, ctx->fork, 0));
-  k_sod_small<<<(unsigned)ctx->num_cus * 8, 64, 0, ctx->side>>>(
-      c, cat->cell_start, (const float4 *)cat->rec, cat->pos, cat->vel,
-      cat->ids, plan, small, large, counters, *out);
-  PSOD_LAUNCH_CHECK();
-  PSOD_HIP(hipEventRecord(ctx->join, ctx->side));
+  // halos' plans and outputs.  They are queued once the search's setup has
+  // run, so that its short kernels do not wait for the small halos' CUs.
+  auto launch_small = [&]() -> int {
+    PSOD_HIP(hipEventRecord(ctx->fork, st));
+    PSOD_HIP(hipStreamWaitEvent(ctx->side, ctx->fork, 0));
+    k_sod_small<<<(unsigned)ctx->num_cus * 8, 64, 0, ctx->side>>>(
+        c, cat->cell_start, (const float4 *)cat->rec, cat->pos, cat->vel,
+        cat->ids, plan, small, large, counters, *out);
+    PSOD_LAUNCH_CHECK();
+    PSOD_HIP(hipEventRecord(ctx->join, ctx->side));
+    return PSOD_OK;
+  };
   for (int i = 0; i < 10; i++) ctx->sod_stat[i] = 0;
   // largest halos first (centre-search tasks and growth), so that the
   // persistent workgroups do not finish on a giant halo picked up last
@@ -2343,7 +2351,9 @@ extern "C" int psod_sod_fof(psod_ctx *ctx, const psod_sod_params *p,
   if (nlarge1 > 0) {
     PSOD_TRY(size_order(nlarge1, 0));
     PSOD_TRY(centre_search(ctx, c, cat, plan, large, nlarge1, ntask, task0,
-                           ntot, hbest));
+                           ntot, hbest, launch_small));
+  } else {
+    PSOD_TRY(launch_small());
   }
   PSOD_HIP(hipStreamWaitEvent(st, ctx->join, 0));
   PSOD_HIP(hipMemcpyAsync(hc, counters, 16, hipMemcpyDeviceToHost, st));
