@@ -1071,6 +1071,29 @@ k_task_items(const int4 *__restrict__ tasks,
   if ((threadIdx.x & 63) == 0 && ns) atomicAdd(&cstat[1], ns);
 }
 
+// Sort keys of the items: the ratio of the group's bound to its halo's best
+// after the pilots, largest first (the groups most likely to raise the best
+// run first; those after them meet a higher best).
+__global__ void k_item_keys(const int32_t *__restrict__ items,
+                            const int32_t *__restrict__ nitems,
+                            const int4 *__restrict__ tasks,
+                            const int32_t *__restrict__ ntask_total,
+                            const unsigned long long *__restrict__ cub,
+                            const unsigned long long *__restrict__ hbest,
+                            uint64_t *__restrict__ keys,
+                            uint32_t *__restrict__ vals) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= *nitems) return;
+  const int32_t nt = *ntask_total, it = items[i];
+  const int32_t t = it >> 3, g = it & 7;
+  const double ub = (double)cub[((int64_t)nt * 27 + t) * 8 + g];
+  const double b = (double)hbest[tasks[t].x];
+  const double r = b > 0.0 ? ub / b : 1e9;
+  const double q = fmin(r, 255.0) * 16777216.0;  // 2^24 per unit of ratio
+  keys[i] = 0xFFFFFFFFull - (uint64_t)q;
+  vals[i] = (uint32_t)it;
+}
+
 // Shell of the cell at position qi of the nearest-first order: the cell
 // itself, the 6 faces, the 12 edges, the 8 corners.
 __device__ inline int cell_shell(int qi) {
@@ -1104,16 +1127,10 @@ k_centre_group(SodCfg c, const int32_t *__restrict__ cell_start,
   const int32_t nt = *ntask_total, ni = *nitems;
   const float cut = c.rs2f * 1.0001f + 1e-6f;  // conservative culling
   const uint64_t below = l == 0 ? 0ull : (~0ull >> (64 - l));  // lanes < l
-  // Blocks of 2 items, visited in a strided order: block k is block
-  // (k % 32) * nb32 + k / 32 of the list, so that the items of one halo
-  // (contiguous in the list) start at times spread over the launch and the
-  // later ones meet the best S raised by the earlier ones.
-  const int32_t nblk = (ni + 1) >> 1, nb32 = (nblk + 31) >> 5;
+  // blocks of 2 items in list order (most promising first)
   for (;;) {
-    const int32_t k = wave_next(wq);
-    if (k >= 32 * nb32) break;
-    const int32_t i0 = 2 * ((k & 31) * nb32 + (k >> 5));
-    if (i0 >= ni) continue;
+    const int32_t i0 = wave_next(wq) * 2;
+    if (i0 >= ni) break;
     const int32_t i1 = min(ni, i0 + 2);
     for (int32_t i = i0; i < i1; i++) {
       const int32_t it = items[i];
@@ -2259,6 +2276,22 @@ static int centre_search(psod_ctx *ctx, const SodCfg &c,
     k_task_items<<<grid_for(nt, 256), 256, 0, st>>>(tasks, ntot, ubt, hpil, cub,
                                                     hbest, items, nitems, cstat);
     PSOD_LAUNCH_CHECK();
+    {
+      int32_t ni = 0;
+      PSOD_HIP(hipMemcpyAsync(&ni, nitems, 4, hipMemcpyDeviceToHost, st));
+      PSOD_HIP(hipStreamSynchronize(st));
+      if (ni > 1) {
+        uint64_t *keys = (uint64_t *)psod_scratch(ctx, S_KEYS, (size_t)ni * 8);
+        uint32_t *vals = (uint32_t *)psod_scratch(ctx, S_VALS, (size_t)ni * 4);
+        if (!keys || !vals) return PSOD_E_NOMEM;
+        k_item_keys<<<grid_for(ni, 256), 256, 0, st>>>(items, nitems, tasks, ntot,
+                                                       cub, hbest, keys, vals);
+        PSOD_LAUNCH_CHECK();
+        PSOD_TRY(psod_radix_sort_u64(ctx, ni, keys, vals, 32));
+        PSOD_HIP(hipMemcpyAsync(items, vals, (size_t)ni * 4,
+                                hipMemcpyDeviceToDevice, st));
+      }
+    }
     k_centre_group<<<resident((const void *)k_centre_group), 256, 0, st>>>(
         c, cat->cell_start, (const float4 *)cat->rec, plan, large, tasks, ntot,
         items, nitems, hbest, cub, ires, cstat, wq + 2);
