@@ -691,10 +691,14 @@ __device__ inline uint64_t group_reduce8(const uint64_t (&u)[8], int l) {
 
 // Phase 0 of the centre search, one wave per task: each candidate's S is at
 // most the sum, over the neighbours of the 27 cells, of the term at the
-// neighbour's distance to the box of the candidate's group of 8.  Stores the
-// bound per (cell, group) in cub[t*27*8 + q*8 + g], the per-group totals in
-// cub[(nt*27 + t)*8 + g], the task's bound in ubt[t] and each halo's largest
-// task bound in hpil.
+// neighbour's distance to the box of the candidate's 16 (candidates
+// 16b..16b+15, Morton-compact).  Boxes of 8 bound tighter but cost twice as
+// many terms; with the items taken most promising first, 16 is faster
+// (c3: 8 -> 236 ms, 16 -> 218, 32 -> 235, 64 -> 263 ms of SOD).  The bound
+// of box b serves both groups of 8 inside it: stores the bound per (cell,
+// group) in cub[t*27*8 + q*8 + g], the per-group totals in
+// cub[(nt*27 + t)*8 + g], the task's bound in ubt[t] and each halo's
+// largest task bound in hpil.
 __global__ void __launch_bounds__(256)
 k_centre_bound(SodCfg c, const int32_t *__restrict__ cell_start,
                const float4 *__restrict__ rec, const HaloPlan *__restrict__ plan,
@@ -744,13 +748,13 @@ k_centre_bound(SodCfg c, const int32_t *__restrict__ cell_start,
       cb1[0] = wave_maxf(have ? px : -INFINITY);
       cb1[1] = wave_maxf(have ? py : -INFINITY);
       cb1[2] = wave_maxf(have ? pz : -INFINITY);
-      // boxes of the 8 groups of 8 candidates (uniform)
+      // boxes of the 4 sets of 16 candidates (uniform)
       float g0x = have ? px : INFINITY, g0y = have ? py : INFINITY,
             g0z = have ? pz : INFINITY;
       float g1x = have ? px : -INFINITY, g1y = have ? py : -INFINITY,
             g1z = have ? pz : -INFINITY;
-  #pragma unroll
-      for (int o = 1; o < 8; o <<= 1) {
+#pragma unroll
+      for (int o = 1; o < 16; o <<= 1) {
         g0x = fminf(g0x, __shfl_xor(g0x, o, 64));
         g0y = fminf(g0y, __shfl_xor(g0y, o, 64));
         g0z = fminf(g0z, __shfl_xor(g0z, o, 64));
@@ -758,17 +762,18 @@ k_centre_bound(SodCfg c, const int32_t *__restrict__ cell_start,
         g1y = fmaxf(g1y, __shfl_xor(g1y, o, 64));
         g1z = fmaxf(g1z, __shfl_xor(g1z, o, 64));
       }
-      float B[8][6];
-  #pragma unroll
-      for (int g = 0; g < 8; g++) {
-        B[g][0] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g0x), 8 * g));
-        B[g][1] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g0y), 8 * g));
-        B[g][2] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g0z), 8 * g));
-        B[g][3] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g1x), 8 * g));
-        B[g][4] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g1y), 8 * g));
-        B[g][5] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g1z), 8 * g));
+      float B[4][6];
+#pragma unroll
+      for (int g = 0; g < 4; g++) {
+        B[g][0] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g0x), 16 * g));
+        B[g][1] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g0y), 16 * g));
+        B[g][2] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g0z), 16 * g));
+        B[g][3] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g1x), 16 * g));
+        B[g][4] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g1y), 16 * g));
+        B[g][5] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g1z), 16 * g));
       }
       const int ngr = (tk.w + 7) >> 3;  // groups holding candidates
+      const int nbx = (tk.w + 15) >> 4;  // boxes holding candidates
       // record ranges of the 27 cells, one per lane, all loaded at once; a
       // cell beyond reach of the candidates' box gets an empty range
       int32_t qa = 0, qb = 0;
@@ -791,11 +796,11 @@ k_centre_bound(SodCfg c, const int32_t *__restrict__ cell_start,
         if (!((rest >> (i >> 3)) & 1)) cb[i] = 0;
       uint64_t tot = 0;    // bound of group l >> 3 over the cells done
       uint64_t cellv = 0;  // the same for the current cell, flushed part
-      // this lane's neighbours of the current cell, per group: at most 255
+      // this lane's neighbours of the current cell, per box: at most 255
       // terms of at most 2^24 + 8 each before a flush
-      uint32_t ubg[8];
-  #pragma unroll
-      for (int g = 0; g < 8; g++) ubg[g] = 0;
+      uint32_t ubg[4];
+#pragma unroll
+      for (int g = 0; g < 4; g++) ubg[g] = 0;
       int nch = 0;  // chunks summed into ubg
       // chunks of 64 records of the cells in turn; the next chunk is requested
       // before the current one is used
@@ -834,9 +839,9 @@ k_centre_bound(SodCfg c, const int32_t *__restrict__ cell_start,
           near = ex * ex + ey * ey + ez * ez <= cut;
         }
         if (near) {
-  #pragma unroll
-          for (int g = 0; g < 8; g++) {
-            if (g < ngr) {
+#pragma unroll
+          for (int g = 0; g < 4; g++) {
+            if (g < nbx) {
               // distance to the box per axis: the coordinate minus its clamp
               const float gx = vx - __builtin_amdgcn_fmed3f(vx, B[g][0], B[g][3]);
               const float gy = vy - __builtin_amdgcn_fmed3f(vy, B[g][1], B[g][4]);
@@ -848,11 +853,12 @@ k_centre_bound(SodCfg c, const int32_t *__restrict__ cell_start,
         }
         if (qn != q || ++nch == 255) {
           uint64_t u[8];
-  #pragma unroll
+#pragma unroll
           for (int g = 0; g < 8; g++) {
-            u[g] = ubg[g];
-            ubg[g] = 0;
+            u[g] = g < ngr ? ubg[g >> 1] : 0;  // both groups of the box
           }
+#pragma unroll
+          for (int g = 0; g < 4; g++) ubg[g] = 0;
           cellv += group_reduce8(u, l);
           nch = 0;
           if (qn != q) {  // the cell is done
@@ -999,7 +1005,7 @@ k_centre_pilot(SodCfg c, const int32_t *__restrict__ cell_start,
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         // m is uniform: a scalar trip count, and a select instead of a branch
         const int32_t mu = __builtin_amdgcn_readfirstlane(m);
-  #pragma unroll 4
+#pragma unroll 4
         for (int32_t u = 0; u < mu; u++) {
           const float4 qv = nbuf[w][u];
           const float dx = qv.x - px, dy = qv.y - py, dz = qv.z - pz;
@@ -1193,7 +1199,7 @@ k_centre_group(SodCfg c, const int32_t *__restrict__ cell_start,
         qbnd = cub[(int64_t)t * 27 * 8 + q * 8 + g];
       }
       // bound of cells qi.. (nearest-first order): suffix sums over lanes
-  #pragma unroll
+#pragma unroll
       for (int o = 1; o < 32; o <<= 1) {
         const unsigned long long v = __shfl_down(qbnd, o, 64);
         if (l + o < 27) qbnd += v;
