@@ -1216,21 +1216,22 @@ k_centre_group(SodCfg c, const int32_t *__restrict__ cell_start,
       uint64_t acc64 = 0, npair = 0;
       bool aborted = false;
       // n queued neighbours from the queue's head, 8 per step
-      auto drain = [&](int n) {
-        if (nst > 255 - 8) {
+      // the queue's first 8 * ns entries, 8 per step (entries past the
+      // queued ones are padding beyond every cutoff)
+      auto pair_step = [&](int j) {
+        const float4 qv = gq[w][j];
+        const float dx = qv.x - px, dy = qv.y - py, dz = qv.z - pz;
+        const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        const uint32_t term = sterm_fix(d2, c.eps2f);
+        part += d2 <= c.rs2f ? term : 0u;
+      };
+      auto fold = [&](int ns) {
+        if (nst > 255 - ns) {
           acc64 += part;
           part = 0;
           nst = 0;
         }
-        nst += (n + 7) >> 3;
-        for (int u = 0; u < n; u += 8) {
-          const int j = u + (l >> 3);
-          const float4 qv = gq[w][j & 127];
-          const float dx = qv.x - px, dy = qv.y - py, dz = qv.z - pz;
-          const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-          const uint32_t term = sterm_fix(d2, c.eps2f);
-          part += (j < n && d2 <= c.rs2f) ? term : 0u;
-        }
+        nst += ns;
       };
       while (qi < 27) {
         const int q = __builtin_amdgcn_readlane(cq, qi);
@@ -1272,21 +1273,30 @@ k_centre_group(SodCfg c, const int32_t *__restrict__ cell_start,
           __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
           __builtin_amdgcn_wave_barrier();
           __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-          const int n = qlen >= 64 ? 64 : qlen;
-          drain(n);
-          // move the entries past the drained ones to the queue's head
-          qlen -= n;
-          float4 mv = make_float4(0.f, 0.f, 0.f, 0.f);
-          if (l < qlen) mv = gq[w][64 + l];
-          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-          __builtin_amdgcn_wave_barrier();
-          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-          if (l < qlen) gq[w][l] = mv;
-          if (shell_end && qlen > 0) {  // at most 63 left: drain them too
+          if (qlen >= 64) {
+            // 64 entries in 8 steps at fixed LDS offsets
+            fold(8);
+#pragma unroll
+            for (int u = 0; u < 8; u++) pair_step(8 * u + (l >> 3));
+            // move the entries past them to the queue's head
+            qlen -= 64;
+            float4 mv = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (l < qlen) mv = gq[w][64 + l];
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            drain(qlen);
+            if (l < qlen) gq[w][l] = mv;
+          }
+          if (shell_end && qlen > 0) {
+            // at most 63 left: pad to a multiple of 8 and take them too
+            const int ns = (qlen + 7) >> 3;
+            if (l >= qlen && l < 8 * ns)
+              gq[w][l] = make_float4(1e30f, 1e30f, 1e30f, 0.f);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            fold(ns);
+            for (int u = 0; u < ns; u++) pair_step(8 * u + (l >> 3));
             qlen = 0;
           }
           __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
