@@ -1034,10 +1034,11 @@ k_centre_pilot(SodCfg c, const int32_t *__restrict__ cell_start,
   }
 }
 
-// Work list of the exact phase: item t*8 + g for every candidate group g of
-// a task that was not a pilot, whose bound reaches the halo's best after the
-// pilots.  A group whose bound is below the best can neither hold the
-// maximum nor tie with it.
+// Work list of the exact phase: item t*8 + g (g even) for every box of 16
+// candidates (groups g and g+1, which share the box's bound) of a task that
+// was not a pilot, whose bound reaches the halo's best after the pilots.  A
+// box whose bound is below the best can neither hold the maximum nor tie
+// with it.
 __global__ void __launch_bounds__(256)
 k_task_items(const int4 *__restrict__ tasks,
              const int32_t *__restrict__ ntask_total,
@@ -1058,7 +1059,7 @@ k_task_items(const int4 *__restrict__ tasks,
       const unsigned long long best = hbest[tk.x];
       const int ngr = (tk.w + 7) >> 3;
       if (ub >= best)
-        for (int g = 0; g < ngr; g++)
+        for (int g = 0; g < ngr; g += 2)
           if (cub[((int64_t)nt * 27 + t) * 8 + g] >= best) mask |= 1u << g;
       skipped = mask == 0;
     }
@@ -1077,7 +1078,7 @@ k_task_items(const int4 *__restrict__ tasks,
   if ((threadIdx.x & 63) == 0 && ns) atomicAdd(&cstat[1], ns);
 }
 
-// Sort keys of the items: the ratio of the group's bound to its halo's best
+// Sort keys of the items: the ratio of the item's bound to its halo's best
 // after the pilots, largest first (the groups most likely to raise the best
 // run first; those after them meet a higher best).
 __global__ void k_item_keys(const int32_t *__restrict__ items,
@@ -1106,15 +1107,17 @@ __device__ inline int cell_shell(int qi) {
   return qi == 0 ? 0 : qi < 7 ? 1 : qi < 19 ? 2 : 3;
 }
 
-// Exact phase, one wave per item (task t, group g).  Lane l takes candidate
-// l & 7 of the group.  The neighbours within reach of the group's box are
+// Exact phase, one wave per item (task t, candidates 8g..8g+15).  Lane l
+// takes candidate l & 15.  The neighbours within reach of the item's box are
 // queued in LDS across chunks and cells and taken 64 at a time, neighbour
-// u + (l >> 3) by lane l in steps of 8, so that every step evaluates 64
+// u + (l >> 4) by lane l in steps of 4, so that every step evaluates 64
 // pairs.  Each pair's term comes from the same operands and operations as in
 // the pilots (neighbour minus candidate, the same fused squares), so S is
 // the same integer.  After each shell of cells (nearest first) the queue is
-// drained and the group is abandoned when no candidate's exact sum so far
-// plus the bound of the cells left reaches the halo's best.
+// drained and the item is abandoned when no candidate's exact sum so far
+// plus the bound of the cells left reaches the halo's best.  Items of 8
+// candidates (the bound's boxes were of 8 then) took 236 ms of SOD at c3
+// against 193 ms with boxes and items of 16.
 __global__ void __launch_bounds__(256)
 k_centre_group(SodCfg c, const int32_t *__restrict__ cell_start,
                const float4 *__restrict__ rec, const HaloPlan *__restrict__ plan,
@@ -1158,8 +1161,8 @@ k_centre_group(SodCfg c, const int32_t *__restrict__ cell_start,
       walk_off(tk.y, 1, &ci, &cj, &ck);
       int32_t lc[3];
       key_at(c.nx, pl.cs, ci, cj, ck, lc);
-      const int nc = min(8, tk.w - 8 * g);  // candidates of the group
-      const int kc = l & 7;                 // this lane's candidate
+      const int nc = min(16, tk.w - 8 * g);  // candidates of the item
+      const int kc = l & 15;                // this lane's candidate
       // lanes of missing candidates sit beyond every cutoff
       float px = 1e30f, py = 1e30f, pz = 1e30f;
       int32_t pidx = 0;
@@ -1170,15 +1173,15 @@ k_centre_group(SodCfg c, const int32_t *__restrict__ cell_start,
         pz = rel1(r.z, lc[2], ck, 0.0f);
         pidx = __float_as_int(r.w);
       }
-      float gb0[3], gb1[3];  // the group's box
+      float gb0[3], gb1[3];  // the item's box
       gb0[0] = wave_minf(kc < nc ? px : INFINITY);
       gb0[1] = wave_minf(kc < nc ? py : INFINITY);
       gb0[2] = wave_minf(kc < nc ? pz : INFINITY);
       gb1[0] = wave_maxf(kc < nc ? px : -INFINITY);
       gb1[1] = wave_maxf(kc < nc ? py : -INFINITY);
       gb1[2] = wave_maxf(kc < nc ? pz : -INFINITY);
-      // record ranges of the 27 cells nearest first and the group's bound in
-      // each, one per lane; a cell beyond reach of the group's box is empty
+      // record ranges of the 27 cells nearest first and the item's bound in
+      // each, one per lane; a cell beyond reach of the item's box is empty
       // (its bound is 0: none of its records is within reach of the group)
       int32_t qa = 0, qb = 0, cq = 0;
       unsigned long long qbnd = 0;
@@ -1275,9 +1278,9 @@ k_centre_group(SodCfg c, const int32_t *__restrict__ cell_start,
           __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
           if (qlen >= 64) {
             // 64 entries in 8 steps at fixed LDS offsets
-            fold(8);
+            fold(16);
 #pragma unroll
-            for (int u = 0; u < 8; u++) pair_step(8 * u + (l >> 3));
+            for (int u = 0; u < 16; u++) pair_step(4 * u + (l >> 4));
             // move the entries past them to the queue's head
             qlen -= 64;
             float4 mv = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1289,14 +1292,14 @@ k_centre_group(SodCfg c, const int32_t *__restrict__ cell_start,
           }
           if (shell_end && qlen > 0) {
             // at most 63 left: pad to a multiple of 8 and take them too
-            const int ns = (qlen + 7) >> 3;
-            if (l >= qlen && l < 8 * ns)
+            const int ns = (qlen + 3) >> 2;
+            if (l >= qlen && l < 4 * ns)
               gq[w][l] = make_float4(1e30f, 1e30f, 1e30f, 0.f);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             fold(ns);
-            for (int u = 0; u < ns; u++) pair_step(8 * u + (l >> 3));
+            for (int u = 0; u < ns; u++) pair_step(4 * u + (l >> 4));
             qlen = 0;
           }
           __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1308,8 +1311,7 @@ k_centre_group(SodCfg c, const int32_t *__restrict__ cell_start,
           acc64 += part;
           part = 0;
           nst = 0;
-          uint64_t v = acc64;  // sum over the 8 lanes of each candidate
-          v += __shfl_xor(v, 8, 64);
+          uint64_t v = acc64;  // sum over the 4 lanes of each candidate
           v += __shfl_xor(v, 16, 64);
           v += __shfl_xor(v, 32, 64);
           const uint32_t blo = __builtin_amdgcn_readlane((uint32_t)qbnd, qn);
@@ -1336,8 +1338,7 @@ k_centre_group(SodCfg c, const int32_t *__restrict__ cell_start,
         continue;
       }
       acc64 += part;
-      uint64_t tot = acc64;  // sum over the 8 lanes of each candidate
-      tot += __shfl_xor(tot, 8, 64);
+      uint64_t tot = acc64;  // sum over the 4 lanes of each candidate
       tot += __shfl_xor(tot, 16, 64);
       tot += __shfl_xor(tot, 32, 64);
       // lanes 0..nc-1 hold the candidates' S
