@@ -57,11 +57,12 @@ int64_t psod_scratch_bytes(psod_ctx *ctx);
 /* counts of the last psod_sod_fof call, first min(n, 10) of: out[0] halos
  * grown by one wave, out[1] halos grown by one workgroup, out[2] of these
  * re-run with larger collection slices, out[3] the keys per slice of that
- * re-run (0 if none), out[4] centre-search tasks evaluated, out[5] tasks
- * skipped by the upper bound, out[6] candidate-neighbour pair tests,
- * out[7] centre-search tasks in all, out[8] centre-search time in ns,
- * out[9] tasks abandoned part way (exact sums plus the bound of the
- * remaining neighbour cells below the best S).
+ * re-run (0 if none), out[4] centre-search units summed to the end (pilot
+ * tasks of 64 candidates and items of 16), out[5] units skipped by their
+ * upper bound, out[6] candidate-neighbour pair tests, out[7] centre-search
+ * tasks in all, out[8] centre-search time in ns, out[9] items abandoned
+ * part way (exact sums plus the bound of the remaining neighbour cells
+ * below the best S).
  * No reference counterpart: diagnostics of the execution paths. */
 int psod_sod_stats(psod_ctx *ctx, int64_t *out, int32_t n);
 

@@ -722,7 +722,7 @@ k_centre_bound(SodCfg c, const int32_t *__restrict__ cell_start,
       const HaloPlan pl = plan[large[tk.x]];
       if (pl.n5 < BB_MIN) {
         // few particles around the seed: every task of the halo is evaluated
-        // exactly in phase 1 (an all-ones bound makes each one a pilot)
+        // exactly by the pilots (an all-ones bound makes each one a pilot)
         if (l == 0) {
           ubt[t] = ~0ull;
           atomicMax(&hpil[tk.x], ~0ull);
