@@ -110,8 +110,10 @@ k_plan(SodCfg c, int64_t nhalo, const int32_t *__restrict__ seed,
        const int32_t *__restrict__ cell_start, HaloPlan *__restrict__ plan,
        int32_t *__restrict__ small, int32_t *__restrict__ large,
        int32_t *__restrict__ counters, psod_halo_out out) {
-  const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (h >= nhalo) return;
+  // one wave per halo: the lanes count the 125 cells around the seed
+  const int l = threadIdx.x & 63;
+  const int64_t h = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (h >= nhalo) return;  // uniform per wave
   HaloPlan pl;
   pl.seed = seed[h];
   pl.nmem = nmem ? nmem[h] : 0;
@@ -124,31 +126,53 @@ k_plan(SodCfg c, int64_t nhalo, const int32_t *__restrict__ seed,
     pl.w0 = 0;
     pl.n5 = 0;
     pl.cs[0] = pl.cs[1] = pl.cs[2] = 0;
-    plan[h] = pl;
-    HaloResult r;
-    zero_result(r);
-    store_result(out, h, c.nbins, r);
-    for (int j = 0; j < c.nbins; j++) out.profile[h * c.nbins + j] = 0;
+    for (int j = l; j < c.nbins; j += 64) out.profile[h * c.nbins + j] = 0;
+    if (l == 0) {
+      plan[h] = pl;
+      HaloResult r;
+      zero_result(r);
+      store_result(out, h, c.nbins, r);
+    }
     return;
   }
   const CellRef sr = ref_of_particle(pos, pl.seed, c.nx);
   int32_t n5 = 0;
-  for (int q = 0; q < 125; q++) {
+  for (int q = l; q < 125; q += 64) {
     int di, dj, dk;
     walk_off(q, 2, &di, &dj, &dk);
-    int32_t l[3];
-    const int32_t key = key_at(c.nx, sr.c, di, dj, dk, l);
+    int32_t lc[3];
+    const int32_t key = key_at(c.nx, sr.c, di, dj, dk, lc);
     n5 += cell_start[key + 1] - cell_start[key];
   }
+  n5 = wave_sum(n5);
+  if (l != 0) return;
   for (int d = 0; d < 3; d++) pl.cs[d] = sr.c[d];
   pl.n5 = n5;
   pl.w0 = reach_estimate(c, pl.nmem);
   pl.klass = (n5 <= CAP5 && pl.w0 <= WMAX_S) ? 0 : 1;
   plan[h] = pl;
-  if (pl.klass == 0)
-    small[atomicAdd(&counters[0], 1)] = (int32_t)h;
-  else
-    large[atomicAdd(&counters[1], 1)] = (int32_t)h;
+}
+
+// the small (class 0) and large (class 1) lists, one atomic per wave and
+// list
+__global__ void __launch_bounds__(256)
+k_plan_lists(int64_t nhalo, const HaloPlan *__restrict__ plan,
+             int32_t *__restrict__ small, int32_t *__restrict__ large,
+             int32_t *__restrict__ counters) {
+  const int l = threadIdx.x & 63;
+  const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = h < nhalo ? plan[h].klass : -1;
+  const uint64_t below = l == 0 ? 0ull : (~0ull >> (64 - l));
+  const uint64_t m0 = __ballot(k == 0), m1 = __ballot(k == 1);
+  int32_t b0 = 0, b1 = 0;
+  if (l == 0) {
+    if (m0) b0 = atomicAdd(&counters[0], __popcll(m0));
+    if (m1) b1 = atomicAdd(&counters[1], __popcll(m1));
+  }
+  b0 = __shfl(b0, 0, 64);
+  b1 = __shfl(b1, 0, 64);
+  if (k == 0) small[b0 + __popcll(m0 & below)] = (int32_t)h;
+  if (k == 1) large[b1 + __popcll(m1 & below)] = (int32_t)h;
 }
 
 // ---------------------------------------------------------------------------
@@ -2347,9 +2371,12 @@ extern "C" int psod_sod_fof(psod_ctx *ctx, const psod_sod_params *p,
   if (!plan || !lists) return PSOD_E_NOMEM;
   int32_t *small = lists, *large = lists + nhalo, *counters = lists + 2 * nhalo;
   PSOD_TRY(psod_fill_i32(ctx, counters, 40, 0));
-  k_plan<<<grid_for(nhalo, 256), 256, 0, st>>>(
+  k_plan<<<grid_for(nhalo, 4), 256, 0, st>>>(
       c, nhalo, d_seed, d_nmem, cat->pos, cat->cell_start, plan, small, large,
       counters, *out);
+  PSOD_LAUNCH_CHECK();
+  k_plan_lists<<<grid_for(nhalo, 256), 256, 0, st>>>(nhalo, plan, small, large,
+                                                     counters);
   PSOD_LAUNCH_CHECK();
   int32_t hc[4];
   PSOD_HIP(hipMemcpyAsync(hc, counters, 16, hipMemcpyDeviceToHost, st));
