@@ -931,6 +931,8 @@ k_centre_pilot(SodCfg c, const int32_t *__restrict__ cell_start,
             w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int32_t nt = *ntask_total;
   const float cut = c.rs2f * 1.0001f + 1e-6f;  // conservative chunk culling
+  // this wave's statistics, added once at the end
+  unsigned long long st_done = 0, st_pairs = 0;
   for (;;) {
     const int32_t t0 = wave_next(wq) * 16;
     if (t0 >= nt) break;
@@ -1051,10 +1053,14 @@ k_centre_pilot(SodCfg c, const int32_t *__restrict__ cell_start,
         taskres[2 * (int64_t)t] = acc;
         taskres[2 * (int64_t)t + 1] = key;
         atomicMax(&hbest[tk.x], (unsigned long long)acc);
-        atomicAdd(&cstat[0], 1ull);
-        atomicAdd(&cstat[2], (unsigned long long)(npair * (uint64_t)tk.w));
+        st_done++;
+        st_pairs += npair * (uint64_t)tk.w;
       }
     }
+  }
+  if (l == 0 && st_done) {
+    atomicAdd(&cstat[0], st_done);
+    atomicAdd(&cstat[2], st_pairs);
   }
 }
 
@@ -1160,6 +1166,8 @@ k_centre_group(SodCfg c, const int32_t *__restrict__ cell_start,
   const int32_t nt = *ntask_total, ni = *nitems;
   const float cut = c.rs2f * 1.0001f + 1e-6f;  // conservative culling
   const uint64_t below = l == 0 ? 0ull : (~0ull >> (64 - l));  // lanes < l
+  // this wave's statistics, added once at the end
+  unsigned long long st_done = 0, st_skip = 0, st_abort = 0, st_pairs = 0;
   // blocks of 2 items in list order (most promising first)
   for (;;) {
     const int32_t i0 = wave_next(wq) * 2;
@@ -1176,7 +1184,7 @@ k_centre_group(SodCfg c, const int32_t *__restrict__ cell_start,
         if (l == 0) {
           res[0] = 0;
           res[1] = ~0ull;
-          atomicAdd(&cstat[1], 1ull);
+          st_skip++;
         }
         continue;
       }
@@ -1356,8 +1364,8 @@ k_centre_group(SodCfg c, const int32_t *__restrict__ cell_start,
         if (l == 0) {
           res[0] = 0;
           res[1] = ~0ull;
-          atomicAdd(&cstat[3], 1ull);
-          atomicAdd(&cstat[2], (unsigned long long)(npair * (uint64_t)nc));
+          st_abort++;
+          st_pairs += npair * (uint64_t)nc;
         }
         continue;
       }
@@ -1374,10 +1382,16 @@ k_centre_group(SodCfg c, const int32_t *__restrict__ cell_start,
         res[0] = acc;
         res[1] = key;
         atomicMax(&hbest[tk.x], (unsigned long long)acc);
-        atomicAdd(&cstat[0], 1ull);
-        atomicAdd(&cstat[2], (unsigned long long)(npair * (uint64_t)nc));
+        st_done++;
+        st_pairs += npair * (uint64_t)nc;
       }
     }
+  }
+  if (l == 0) {
+    if (st_done) atomicAdd(&cstat[0], st_done);
+    if (st_skip) atomicAdd(&cstat[1], st_skip);
+    if (st_pairs) atomicAdd(&cstat[2], st_pairs);
+    if (st_abort) atomicAdd(&cstat[3], st_abort);
   }
 }
 
