@@ -238,6 +238,11 @@ __global__ void k_cs_split(const int32_t *__restrict__ O, int dbits,
   bs2[B] = ntb[b] > 0 ? O[(int64_t)tb[b] * nd + (int64_t)d * ntb[b]] : bs[b];
 }
 
+// Dense cells (at least DENSE records) are reordered after the sort; those
+// of at most SUB_REG records fit the registers of one workgroup.
+constexpr int DENSE = 64;
+constexpr int SUB_T = 256, SUB_V = 16, SUB_REG = SUB_T * SUB_V;
+
 // One workgroup per block: counting sort of its records by local cell, the
 // final records and the block's 512 cell offsets.  A block of at most BS_CAP
 // records is read once into registers, placed in LDS in cell order and
@@ -255,7 +260,9 @@ k_block_sort(const int32_t *__restrict__ bstart, int64_t nblk,
              const float4 *__restrict__ stage,
              const int32_t *__restrict__ stage_lab,
              int32_t *__restrict__ cell_start, float4 *__restrict__ rec,
-             int32_t *__restrict__ perm, int32_t *__restrict__ lab_sorted) {
+             int32_t *__restrict__ perm, int32_t *__restrict__ lab_sorted,
+             int32_t *__restrict__ dlist, int32_t *__restrict__ dbig,
+             int32_t *__restrict__ dcnt, int32_t dcap) {
   __shared__ int32_t cnt[512];
   __shared__ float4 sr[BS_CAP];
   __shared__ int32_t sl[BS_CAP];
@@ -286,10 +293,34 @@ k_block_sort(const int32_t *__restrict__ bstart, int64_t nblk,
       for (int k = 0; k < 8; k++) { v[k] = cnt[threadIdx.x * 8 + k]; s += v[k]; }
       const int32_t inc = wave_incl_scan(s);
       int32_t e = inc - s;
+      int nr = 0, ng = 0;  // this lane's dense cells, by list
       for (int k = 0; k < 8; k++) {
         cnt[threadIdx.x * 8 + k] = e;
         cell_start[b * 512 + threadIdx.x * 8 + k] = a + e;
         e += v[k];
+        nr += v[k] >= DENSE && v[k] <= SUB_REG;
+        ng += v[k] > SUB_REG;
+      }
+      // dense cells to their lists: one atomic per block and list
+      const int32_t ir = wave_incl_scan(nr), ig = wave_incl_scan(ng);
+      const int32_t tr = __shfl(ir, 63, 64), tg = __shfl(ig, 63, 64);
+      const int32_t er = ir - nr, eg = ig - ng;
+      int32_t br = 0, bg = 0;
+      if (threadIdx.x == 0) {
+        if (tr > 0) br = atomicAdd(&dcnt[0], tr);
+        if (tg > 0) bg = atomicAdd(&dcnt[2], tg);
+      }
+      br = __shfl(br, 0, 64) + er;
+      bg = __shfl(bg, 0, 64) + eg;
+      for (int k = 0; k < 8; k++) {
+        const int32_t cell = (int32_t)(b * 512 + threadIdx.x * 8 + k);
+        if (v[k] >= DENSE && v[k] <= SUB_REG) {
+          if (br < dcap) dlist[br] = cell;
+          br++;
+        } else if (v[k] > SUB_REG) {
+          if (bg < dcap) dbig[bg] = cell;
+          bg++;
+        }
       }
     }
     __syncthreads();
@@ -336,8 +367,6 @@ int psod_fill_i32(psod_ctx *c, int32_t *p, int64_t n, int32_t v);
 // meaning elsewhere; the centre search uses the compact chunks to skip
 // neighbour chunks beyond its 1-cell reach.
 // ---------------------------------------------------------------------------
-constexpr int DENSE = 64;
-constexpr int SUB_T = 256, SUB_V = 16, SUB_REG = SUB_T * SUB_V;
 constexpr int64_t SUB_TMP = 1 << 18;  // records per workgroup slice
 
 __device__ inline uint32_t spread3(uint32_t v) {
@@ -348,36 +377,6 @@ __device__ inline uint32_t morton9(const float4 &r) {
   const uint32_t qy = (uint32_t)((r.y - floorf(r.y)) * 8.0f) & 7;
   const uint32_t qz = (uint32_t)((r.z - floorf(r.z)) * 8.0f) & 7;
   return spread3(qx) | (spread3(qy) << 1) | (spread3(qz) << 2);
-}
-
-// dense cells that fit the registers of one workgroup go to list[0..),
-// larger ones to big[0..); cnt[0] and cnt[2] count them (one atomic per
-// wave and list)
-__global__ void k_dense_cells(const int32_t *__restrict__ cs, int64_t ncell,
-                              int32_t *__restrict__ list,
-                              int32_t *__restrict__ big,
-                              int32_t *__restrict__ cnt, int32_t cap) {
-  const int l = threadIdx.x & 63;
-  const uint64_t below = l == 0 ? 0ull : (~0ull >> (64 - l));
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t c0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63);
-       c0 < ncell; c0 += stride) {
-    const int64_t c = c0 + l;
-    const int32_t n = c < ncell ? cs[c + 1] - cs[c] : 0;
-    const bool dn = n >= DENSE, reg = n <= SUB_REG;
-    const uint64_t m0 = __ballot(dn && reg), m1 = __ballot(dn && !reg);
-    int32_t b0 = 0, b1 = 0;
-    if (l == 0) {
-      if (m0) b0 = atomicAdd(&cnt[0], __popcll(m0));
-      if (m1) b1 = atomicAdd(&cnt[2], __popcll(m1));
-    }
-    b0 = __shfl(b0, 0, 64);
-    b1 = __shfl(b1, 0, 64);
-    if (dn) {
-      const int32_t k = reg ? b0 + __popcll(m0 & below) : b1 + __popcll(m1 & below);
-      if (k < cap) (reg ? list : big)[k] = (int32_t)c;
-    }
-  }
 }
 
 __global__ void __launch_bounds__(SUB_T)
@@ -551,25 +550,24 @@ extern "C" int psod_cell_sort(psod_ctx *ctx, int64_t npart, const float *d_pos,
   const int32_t *bstart = bs;  // nblk + 1 block starts
   const float4 *stage = recb[cur];
   const int32_t *slab = labb[cur];
+  // lists of the dense cells, filled by the block sort: ctl[0] and ctl[2]
+  // count them, ctl[1] and ctl[3] are the subsort's queues (at most
+  // npart / DENSE cells are dense)
+  const int32_t cap = (int32_t)std::min<int64_t>(ncell, npart / DENSE + 32);
+  int32_t *list = (int32_t *)psod_scratch(ctx, S_HIST,
+                                          (size_t)(2 * (int64_t)cap + 64) * 4);
+  int32_t *ctl = (int32_t *)psod_scratch(ctx, S_MISC, 64);
+  if (!list || !ctl) return PSOD_E_NOMEM;
+  int32_t *big = list + cap + 32;
+  PSOD_TRY(psod_fill_i32(ctx, ctl, 4, 0));
   k_block_sort<<<grid_for(nblk, 1, (int64_t)ctx->num_cus * 32), 256, 0, st>>>(
       bstart, nblk, stage, slab, d_cell_start, (float4 *)d_rec, d_perm,
-      d_lab_sorted);
+      d_lab_sorted, list, big, ctl, cap);
   PSOD_LAUNCH_CHECK();
   k_set_last<<<1, 64, 0, st>>>(d_cell_start, ncell, (int32_t)npart);
   PSOD_LAUNCH_CHECK();
   if (npart > 0) {
     // Morton order of the 1/8-cell sub-cells inside dense cells
-    // (at most npart / DENSE cells are dense)
-    const int32_t cap = (int32_t)std::min<int64_t>(ncell, npart / DENSE + 32);
-    int32_t *list = (int32_t *)psod_scratch(ctx, S_HIST,
-                                            (size_t)(2 * (int64_t)cap + 64) * 4);
-    int32_t *ctl = (int32_t *)psod_scratch(ctx, S_MISC, 64);
-    if (!list || !ctl) return PSOD_E_NOMEM;
-    int32_t *big = list + cap + 32;
-    PSOD_TRY(psod_fill_i32(ctx, ctl, 4, 0));
-    k_dense_cells<<<grid_for(ncell, 256, (int64_t)ctx->num_cus * 64), 256, 0,
-                    st>>>(d_cell_start, ncell, list, big, ctl, cap);
-    PSOD_LAUNCH_CHECK();
     // cells that fit the registers: many workgroups, no staging slices;
     // larger cells: one workgroup per CU, each with a staging slice
     k_subsort<<<(unsigned)ctx->num_cus * 8, SUB_T, 0, st>>>(
