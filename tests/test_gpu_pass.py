@@ -260,6 +260,12 @@ def test_sod_fof_c1(ctx, orc):
     check_sod(o, r, 1)
     # both execution paths were exercised
     assert (o["status"] & 4).any() and (~o["status"] & 4).any()
+    # and the centre search's branch and bound: tasks bounded, boxes skipped
+    # by their bound, boxes summed as items or abandoned part-way
+    st = ctx.sod_stats()
+    assert st["centre_tasks_all"] > 0 and st["pair_tests"] > 0
+    assert st["centre_skipped"] > 0
+    assert st["centre_aborted"] > 0
 
 
 @pytest.mark.parametrize("name", ["shells_crossing.npz", "shells_floor.npz"])
