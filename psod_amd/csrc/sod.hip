@@ -2223,10 +2223,10 @@ __global__ void k_retry_reset(int32_t *__restrict__ c, int32_t nitem,
 // cells).  For the growth it is the particles of the cube of the planned
 // reach, summed over the 8^3 blocks that the cube overlaps: a small halo
 // beside a giant one walks the giant's particles too.
-__global__ void k_large_keys(const HaloPlan *__restrict__ plan,
+__global__ void k_large_keys(SodCfg c, const HaloPlan *__restrict__ plan,
                              const int32_t *__restrict__ large, int32_t n,
                              const int32_t *__restrict__ cell_start,
-                             int32_t nx, int growth,
+                             int growth,
                              uint64_t *__restrict__ keys,
                              uint32_t *__restrict__ vals) {
   const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2235,7 +2235,7 @@ __global__ void k_large_keys(const HaloPlan *__restrict__ plan,
   const HaloPlan pl = plan[h];
   int64_t cost = (int64_t)max(pl.nmem, 0) + pl.n5;
   if (growth) {
-    const int32_t nb = nx >> 3, W = max(pl.w0, 2);
+    const int32_t nb = c.nx >> 3, W = max(pl.w0, 2);
     int32_t lo[3], hi[3];
     for (int d = 0; d < 3; d++) {
       lo[d] = floor8(pl.cs[d] - W);
@@ -2249,6 +2249,13 @@ __global__ void k_large_keys(const HaloPlan *__restrict__ plan,
                             wrapi(bx, nb);
           cost += cell_start[(b + 1) * BS3] - cell_start[b * BS3];
         }
+    // A cube already holding more particles than a sphere of radius
+    // W / rmax_frac at the first threshold would is likely to be outgrown
+    // and walked again at twice the reach (about 4x the particles): such a
+    // halo is costed at both walks, so that it starts among the first.
+    const double rw = (double)W / c.rmax_frac;
+    if ((double)cost * c.mpart > c.delta[0] * (4.0 / 3.0 * M_PI) * rw * rw * rw)
+      cost *= 5;
   }
   keys[e] = (uint64_t)(0x7FFFFFFFFFFFll - min(cost, (int64_t)0x7FFFFFFFFFFFll));
   vals[e] = (uint32_t)h;
@@ -2420,7 +2427,7 @@ extern "C" int psod_sod_fof(psod_ctx *ctx, const psod_sod_params *p,
     uint32_t *vals = (uint32_t *)psod_scratch(ctx, S_VALS, (size_t)n * 4);
     if (!keys || !vals) return PSOD_E_NOMEM;
     k_large_keys<<<grid_for(n, 256), 256, 0, st>>>(
-        plan, large, n, cat->cell_start, c.nx, growth, keys, vals);
+        c, plan, large, n, cat->cell_start, growth, keys, vals);
     PSOD_LAUNCH_CHECK();
     PSOD_TRY(psod_radix_sort_u64(ctx, n, keys, vals, 48));
     PSOD_HIP(hipMemcpyAsync(large, vals, (size_t)n * 4,
