@@ -1453,6 +1453,8 @@ struct GrowShared {
   uint64_t skey[SORT_CAP];
   int32_t pre[GL_T + 1];
   int32_t st[GL_T];
+  int32_t sdi[GL_T];  // per segment: di of the cell at x = 0 of its block
+  int32_t spk[GL_T];  // per segment: dj, dk (12 bits each, +2048), y & 7, z & 7
   int32_t red_i[GL_T / 64];
   int64_t red_l[GL_T / 64];
   double red_d[GL_T / 64];
@@ -1605,13 +1607,17 @@ __device__ inline void cube_walk(const SodCfg &c, GrowShared &G,
   const CubeSegs g = cube_segs(c, cr, W);
   for (int64_t base = 0; base < g.nseg; base += GL_T) {
     const int64_t q = base + threadIdx.x;
-    int32_t cnt = 0, st = 0;
+    int32_t cnt = 0, st = 0, sdi = 0, spk = 0;
     if (q < g.nseg) {
       int32_t bx, gy, gz, ka, ncx;
       int dj, dk;
       cube_seg(c, cr, g, W, (int32_t)q, &bx, &dj, &dk, &gy, &gz, &ka, &ncx);
       st = cell_start[ka];
       cnt = cell_start[ka + ncx] - st;
+      // |dj|, |dk| <= W < 2048 (2W + 1 <= nx)
+      sdi = 8 * bx - cr.c[0];
+      spk = (dj + 2048) | ((dk + 2048) << 12) | ((gy & 7) << 24) |
+            ((gz & 7) << 27);
     }
     // block exclusive scan of cnt
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -1626,6 +1632,8 @@ __device__ inline void cube_walk(const SodCfg &c, GrowShared &G,
     }
     G.pre[threadIdx.x] = off + ex;
     G.st[threadIdx.x] = st;
+    G.sdi[threadIdx.x] = sdi;
+    G.spk[threadIdx.x] = spk;
     if (threadIdx.x == 0) G.pre[GL_T] = all;
     __syncthreads();
     const int nq = (int)min((int64_t)GL_T, g.nseg - base);
@@ -1647,13 +1655,11 @@ __device__ inline void cube_walk(const SodCfg &c, GrowShared &G,
       for (int u = 0; u < CW_U; u++) {
         if (j0 + u * GL_T >= all) continue;
         const float4 r = rv[u];
-        int32_t bx, gy, gz, ka, ncx;
-        int dj, dk;
-        cube_seg(c, cr, g, W, (int32_t)(base + qq[u]), &bx, &dj, &dk, &gy,
-                 &gz, &ka, &ncx);
+        const int32_t pk = G.spk[qq[u]];
         const int32_t lx = (int32_t)r.x;  // u in [0, 8): the cell in the block
-        const int di = 8 * bx + lx - cr.c[0];
-        int32_t lc[3] = {lx, gy & 7, gz & 7};
+        const int di = G.sdi[qq[u]] + lx;
+        const int dj = (pk & 4095) - 2048, dk = ((pk >> 12) & 4095) - 2048;
+        int32_t lc[3] = {lx, (pk >> 24) & 7, (pk >> 27) & 7};
         const float dx = rel1(r.x, lc[0], di, cr.f[0]);
         const float dy = rel1(r.y, lc[1], dj, cr.f[1]);
         const float dz = rel1(r.z, lc[2], dk, cr.f[2]);
