@@ -1598,12 +1598,56 @@ __device__ inline int64_t cube_count(const SodCfg &c, GrowShared &G,
 
 constexpr int CW_U = 4;
 
-// Calls f(d2, idx, rec, di, dj, dk, lc) for every particle of the cells at
-// offsets [-W, W]^3 from the reference cell.
+// Whether a row segment (cell offsets dia..dib along x, dj, dk) can hold a
+// particle whose squared distance from the reference lies in [lo2, hi2].
+// A particle of the cell at offset d lies in [d - f, d + 1 - f) along each
+// axis; the intervals are widened by 1e-3 cells for the float rounding of
+// the distances.
+__device__ inline bool seg_meets(const CellRef &cr, int dia, int dib, int dj,
+                                 int dk, float lo2, float hi2) {
+  float mn = 0.0f, mx = 0.0f;
+  auto axis = [&](float a, float b) {
+    a -= 1e-3f;
+    b += 1e-3f;
+    const float m = a > 0.0f ? a : (b < 0.0f ? -b : 0.0f);
+    const float M = fmaxf(fabsf(a), fabsf(b));
+    mn += m * m;
+    mx += M * M;
+  };
+  axis((float)dia - cr.f[0], (float)(dib + 1) - cr.f[0]);
+  axis((float)dj - cr.f[1], (float)(dj + 1) - cr.f[1]);
+  axis((float)dk - cr.f[2], (float)(dk + 1) - cr.f[2]);
+  return mn <= hi2 && mx >= lo2;
+}
+
+// Band of squared distances [*lo2, *hi2] that holds every particle of
+// d2 < W2 whose rbin() lies in [b0, b1], with a 1e-4 relative margin.
+__device__ inline void bin_band(int32_t b0, int32_t b1, int32_t nb,
+                                float scale, float W2, float *lo2,
+                                float *hi2) {
+  const double s = (double)scale;
+  if (b0 > 0) {
+    const double c0 = cbrt((double)b0 / s * (1.0 - 1e-4));
+    *lo2 = (float)(c0 * c0);
+  } else {
+    *lo2 = -1.0f;
+  }
+  if (b1 >= nb - 1) {
+    *hi2 = W2 * (1.0f + 1e-4f) + 1e-3f;
+  } else {
+    const double c1 = cbrt((double)(b1 + 1) / s * (1.0 + 1e-4));
+    *hi2 = (float)(c1 * c1) + 1e-6f;
+  }
+}
+
+// Calls f(d2, idx, rec, di, dj, dk, lc) for the particles of the cells at
+// offsets [-W, W]^3 from the reference cell, skipping the row segments that
+// cannot hold a particle with d2 in [lo2, hi2] (f sees all the others).
 template <typename F>
 __device__ inline void cube_walk(const SodCfg &c, GrowShared &G,
                                  const int32_t *cell_start, const float4 *rec,
-                                 const CellRef &cr, int W, F &&f) {
+                                 const CellRef &cr, int W, float lo2,
+                                 float hi2, F &&f) {
   const CubeSegs g = cube_segs(c, cr, W);
   for (int64_t base = 0; base < g.nseg; base += GL_T) {
     const int64_t q = base + threadIdx.x;
@@ -1612,8 +1656,11 @@ __device__ inline void cube_walk(const SodCfg &c, GrowShared &G,
       int32_t bx, gy, gz, ka, ncx;
       int dj, dk;
       cube_seg(c, cr, g, W, (int32_t)q, &bx, &dj, &dk, &gy, &gz, &ka, &ncx);
-      st = cell_start[ka];
-      cnt = cell_start[ka + ncx] - st;
+      const int dia = max(g.X0, 8 * bx) - cr.c[0];
+      if (seg_meets(cr, dia, dia + ncx - 1, dj, dk, lo2, hi2)) {
+        st = cell_start[ka];
+        cnt = cell_start[ka + ncx] - st;
+      }
       // |dj|, |dk| <= W < 2048 (2W + 1 <= nx)
       sdi = 8 * bx - cr.c[0];
       spk = (dj + 2048) | ((dk + 2048) << 12) | ((gy & 7) << 24) |
@@ -1795,18 +1842,18 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
       __threadfence_block();
       __syncthreads();
       // ---- pass 1: histogram -------------------------------------------
-      int64_t ng = 0;
-      cube_walk(c, G, cell_start, rec, cr, W,
+      // every particle of the cube is gathered; those of d2 < W2 are binned
+      const int64_t ng = ncube;
+      cube_walk(c, G, cell_start, rec, cr, W, -1.0f,
+                W2 * (1.0f + 1e-4f) + 1e-3f,
                 [&](float d2, int32_t, const float4 &, int, int, int,
                     const int32_t *) {
-                  ng++;
                   if (d2 < W2) {
                     const int32_t b = rbin(d2, scale, nb);
                     if (b < HIST_LDS) atomicAdd(&lh[b], 1);
                     else atomicAdd(&hist[b], 1);
                   }
                 });
-      ng = block_sum_i64(ng, G);
       for (int32_t b = threadIdx.x; b < nl; b += GL_T) hist[b] = lh[b];
       __threadfence();
       __syncthreads();
@@ -1965,7 +2012,9 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
       const int nr = G.nr;
       if (G.need > pool_cap) { failed = true; break; }
       // ---- pass 2: collect the ranges ---------------------------------------
-      cube_walk(c, G, cell_start, rec, cr, W,
+      float blo2 = -1.0f, bhi2 = -2.0f;  // nothing when there is no range
+      if (nr > 0) bin_band(G.rs[0], G.re[nr - 1], nb, scale, W2, &blo2, &bhi2);
+      cube_walk(c, G, cell_start, rec, cr, W, blo2, bhi2,
                 [&](float d2, int32_t idx, const float4 &, int, int, int,
                     const int32_t *) {
                   if (!(d2 < W2)) return;
@@ -2105,7 +2154,20 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
       double sp[3] = {0, 0, 0}, sv[3] = {0, 0, 0};
       int64_t nbd = 0;
       const int flagv = (c.flags & PSOD_F_VELOCITY) && vel;
-      cube_walk(c, G, cell_start, rec, cr, W,
+      // farthest particle this pass uses: profile, boundary band, centre of
+      // mass (keys up to keyR) and the second threshold's bins
+      float phi2;
+      {
+        const double rm = fmax(rlast, R0 + c.tol_abs);
+        float h2 = fmaxf((float)(rm * rm * (1.0 + 1e-4)) + 1e-6f, key_d2(keyR));
+        for (int j = 0; j < ns2; j++) {
+          float l2, u2;
+          bin_band(G.rs[R2 + j], G.re[R2 + j], nb, scale, W2, &l2, &u2);
+          h2 = fmaxf(h2, u2);
+        }
+        phi2 = fminf(h2, W2 * (1.0f + 1e-4f) + 1e-3f);
+      }
+      cube_walk(c, G, cell_start, rec, cr, W, -1.0f, phi2,
                 [&](float d2, int32_t idx, const float4 &r, int di, int dj,
                     int dk, const int32_t *lc) {
                   if (!(d2 < W2)) return;
