@@ -248,7 +248,7 @@ constexpr int SUB_T = 256, SUB_V = 16, SUB_REG = SUB_T * SUB_V;
 // records is read once into registers, placed in LDS in cell order and
 // written out as one contiguous run; a larger one is read twice (count, then
 // place) and written record by record.
-constexpr int BS_CAP = 2048;
+constexpr int BS_CAP = 1024;  // 22 KB of LDS: 6 workgroups per CU (by registers)
 constexpr int BS_U = BS_CAP / 256;
 
 __device__ inline int32_t local_cell(const float4 &r) {
