@@ -112,6 +112,21 @@ class Context:
         return self.torch.empty(shape, dtype=dtype,
                                 device=f"cuda:{self.device}")
 
+    def _after_torch(self):
+        """Orders the context's stream after the work queued so far on
+        torch's current stream.  The library's streams are non-blocking, so
+        without this a tensor that torch fills or copies just before a call
+        (the zeroed seed keys, a caller's upload) could still be written by
+        torch while the library's kernels use it.  The entry points
+        synchronise before returning, so their outputs need no such step."""
+        torch = self.torch
+        ext = getattr(self, "_ext", None)
+        if ext is None:
+            ext = torch.cuda.ExternalStream(
+                self.stream, device=torch.device("cuda", self.device))
+            self._ext = ext
+        ext.wait_stream(torch.cuda.current_stream(self.device))
+
     # ---- synthetic catalogue -------------------------------------------
     def generate(self, g, with_vel=True):
         torch = self.torch
@@ -121,6 +136,7 @@ class Context:
         vel = self._empty((n, 3), torch.float32) if with_vel else None
         lab = self._empty((n,), torch.int32)
         ids = self._empty((n,), torch.int64)
+        self._after_torch()
         _lib.check(_L.psod_generate(
             self.h, C.byref(g), _ptr(t["offset"]), _ptr(t["centre"]),
             _ptr(t["conc"]), _ptr(t["r200"]), _ptr(t["bulk"]), _ptr(pos),
@@ -135,6 +151,7 @@ class Context:
         perm = self._empty((n,), torch.int32) if want_perm else None
         rec = self._empty((n, 4), torch.float32)
         labs = self._empty((n,), torch.int32) if labels is not None else None
+        self._after_torch()
         _lib.check(_L.psod_cell_sort(self.h, n, _ptr(pos), _ptr(labels), nx,
                                      _ptr(cs), _ptr(perm), _ptr(rec),
                                      _ptr(labs)))
@@ -145,6 +162,7 @@ class Context:
     def cic_deposit(self, cat, ng, shift=0, out=None):
         grid = out if out is not None else self._empty((ng, ng, ng),
                                                        self.torch.float32)
+        self._after_torch()
         _lib.check(_L.psod_cic_deposit(self.h, cat["npart"], cat["nx"],
                                        _ptr(cat["cell_start"]),
                                        _ptr(cat["rec"]), ng, shift,
@@ -157,6 +175,7 @@ class Context:
         n = cat["npart"]
         rs = self._empty((n,), torch.float32) if sorted_out else None
         ro = self._empty((n,), torch.float32) if orig_out else None
+        self._after_torch()
         _lib.check(_L.psod_cic_interp(
             self.h, n, cat["nx"], _ptr(cat["cell_start"]), _ptr(cat["rec"]),
             grid.shape[0], shift, _ptr(grid), _ptr(rs), _ptr(ro),
@@ -170,11 +189,13 @@ class Context:
         self.cic_interp(cat, grid, shift=shift, orig_out=False, nhalo=nhalo,
                         seed_key=key)
         seed = self._empty((nhalo,), self.torch.int32)
+        self._after_torch()
         _lib.check(_L.psod_seed_index(self.h, nhalo, _ptr(key), _ptr(seed)))
         return seed
 
     def label_counts(self, labels, nhalo):
         out = self._empty((nhalo,), self.torch.int32)
+        self._after_torch()
         _lib.check(_L.psod_label_counts(self.h, labels.shape[0], _ptr(labels),
                                         nhalo, _ptr(out)))
         return out
@@ -184,6 +205,7 @@ class Context:
         n = rho.shape[0]
         isort = self._empty((n,), self.torch.int32)
         nn = C.c_int64(0)
+        self._after_torch()
         _lib.check(_L.psod_density_order(self.h, n, _ptr(rho), threshold,
                                          _ptr(isort), C.byref(nn)))
         return isort, nn.value
@@ -222,6 +244,7 @@ class Context:
                            _ptr(cat["cell_start"]).value,
                            _ptr(cat["rec"]).value)
         f = _L.psod_sod_fof_plain if plain else _L.psod_sod_fof
+        self._after_torch()
         _lib.check(f(self.h, C.byref(p), C.byref(c), nh, _ptr(seeds),
                      _ptr(nmem), C.byref(ho)))
         return o
@@ -234,6 +257,7 @@ class Context:
         ng = nx if ng is None else ng
         o = out if out is not None else self.alloc_halo_out(nhalo, p.nbins)
         ho = _lib.HaloOut(**{k: _ptr(o[k]).value for k in o})
+        self._after_torch()
         _lib.check(_L.psod_pass(self.h, C.byref(p), pos.shape[0], nx, ng,
                                 _ptr(pos), _ptr(vel), _ptr(ids), _ptr(labels),
                                 nhalo, _ptr(grid), C.byref(ho)))
