@@ -105,6 +105,26 @@ def test_cic_deposit_and_interp(ctx, orc, c64, ratio, shift_frac):
     np.testing.assert_allclose(host(rho), orho, rtol=1e-6)
 
 
+def test_seeds_ordered_after_torch_work(ctx, c64):
+    """The seed keys are zeroed on torch's stream and filled by the
+    library's stream.  Several milliseconds of matrix products queued on
+    torch's stream delay the zeroing, so unordered streams would lose keys;
+    the seeds must match those of an unloaded call."""
+    import torch
+    nx = c64["nx"]
+    cat = ctx.cell_sort(c64["pos"], nx, labels=c64["lab"])
+    grid = ctx.cic_deposit(cat, nx)
+    ref = host(ctx.seeds(cat, grid, c64["nh"]))
+    a = torch.rand((4096, 4096), device="cuda:0")
+    for _ in range(8):
+        a = torch.mm(a, a) * (1.0 / 4096)
+    got = host(ctx.seeds(cat, grid, c64["nh"]))
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(got, ref)
+    lab = host(c64["lab"])
+    assert np.all(lab[got] == np.arange(c64["nh"]))
+
+
 def test_seeds_and_density_order(ctx, orc, c64):
     nx = c64["nx"]
     cat = ctx.cell_sort(c64["pos"], nx, labels=c64["lab"])
