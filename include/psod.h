@@ -98,8 +98,9 @@ int psod_cic_deposit(psod_ctx *ctx, int64_t npart, int32_t nx,
  * Interpolates the grid back to every particle.  Optional outputs:
  *   d_rho_sorted [npart] in sorted order, d_rho [npart] in original order,
  *   d_seed_key [nhalo]: for each label h >= 0 the member with the largest rho
- *   (ties: smallest index) packed as (float bits of rho << 32 | ~index); the
- *   caller zeroes it before the first call. */
+ *   (ties: smallest index) packed as (float bits of rho << 32 | ~index), 0
+ *   for a label without members.  The call clears the keys itself on its
+ *   own stream before it fills them. */
 int psod_cic_interp(psod_ctx *ctx, int64_t npart, int32_t nx,
                     const int32_t *d_cell_start, const float *d_rec,
                     int32_t ng, int32_t shift, const float *d_grid,
@@ -140,6 +141,10 @@ typedef struct {
 } psod_sod_params;
 
 #define PSOD_F_VELOCITY 1  /* d_vel is given: centre-of-mass velocity        */
+#define PSOD_F_CENTRE_ONLY 2 /* psod_sod_fof_plain: the centre search alone;
+                              * growth outputs are zero (no reference
+                              * counterpart: the FP64 centre of halos whose
+                              * plain growth would take minutes)            */
 
 /* Per-halo outputs (device arrays, nhalo rows). */
 typedef struct {
@@ -236,18 +241,34 @@ int psod_generate(psod_ctx *ctx, const psod_gen_params *g,
  * Each replaces one reference subroutine for a single zone covering the box.
  */
 
-/* `cic` (main file:1077-1109): equal masses (mass may be NULL or all equal),
- * zone origin xxmin with xxmin*ng integral.  grid [ng^3] float32. */
+/* The reference's cube and rho are real(8) (main file:22-23): these three
+ * compute in FP64 with the reference's expressions and operation order.
+ * Node sums are FP64 atomic adds (they differ from the reference's
+ * particle-order sums by the rounding of the additions only). */
+
+/* `cic` (main file:1077-1109, deposit :1133-1215, normalise :1263): equal
+ * masses (mass may be NULL, giving the reference's mpf = 1/N in real(4),
+ * or all equal), nodes at xxmin + i/ng, periodic wrap at ng.
+ * grid [ng^3] FP64, x fastest. */
 int psod_host_cic(psod_ctx *ctx, int64_t npart, const float *pos,
-                  const float *mass, double xxmin, int32_t ng, float *grid);
+                  const float *mass, double xxmin, int32_t ng, double *grid);
 
-/* `cicm1` (main file:1268-1285): rho [npart] at each particle. */
+/* `cicm1` (main file:1268-1390): rho [npart] FP64 at each particle. */
 int psod_host_cicm1(psod_ctx *ctx, int64_t npart, const float *pos,
-                    double xxmin, int32_t ng, const float *grid, float *rho);
+                    double xxmin, int32_t ng, const double *grid,
+                    double *rho);
 
-/* `tri_cic` (main file:998-1006): isort [npart] 1-based, decreasing rho. */
-int psod_host_tri_cic(psod_ctx *ctx, int64_t npart, const float *rho,
+/* `tri_cic` (main file:998-1037): isort [npart] 1-based by ascending
+ * pdens = 1/rho (FP64), ties by index; *npart_new = first 1-based rank
+ * with pdens >= 1/threshold (at most npart). */
+int psod_host_tri_cic(psod_ctx *ctx, int64_t npart, const double *rho,
                       double threshold, int32_t *isort, int64_t *npart_new);
+
+/* Device form of `tri_cic` on FP64 densities (main file:1022-1032), the
+ * order psod_host_tri_cic returns. */
+int psod_density_order_f64(psod_ctx *ctx, int64_t npart, const double *d_rho,
+                           double threshold, int32_t *d_isort,
+                           int64_t *npart_new);
 
 /* `spherover` (main file:1396-1400), greedy seeds-with-exclusion semantics.
  * Positions are UNscaled box units; nx, ncut, seuil, seuil2, mmin, nsel as the

@@ -127,6 +127,9 @@ def _declare(L):
     L.or_sod_fof.argtypes = [i64, v, v, v, v, v, i32, i64, v, v,
                              C.POINTER(FofParams), v, v]
     L.or_sod_fof.restype = C.c_int
+    L.or_sod_fof_at.argtypes = [i64, v, v, v, v, v, i32, i64, v, v, v,
+                                C.POINTER(FofParams), v, v]
+    L.or_sod_fof_at.restype = C.c_int
     L.or_computecom.argtypes = [i64, v, v, v, v]
     L.or_delta_crit_to_mean.argtypes = [d, d]
     L.or_delta_crit_to_mean.restype = d
@@ -282,7 +285,10 @@ def fof_seeds(labels, rho, nhalo):
 
 
 def sod_fof(pos, cell_start, perm, bs, seeds, nmem, params, vel=None,
-            ids=None):
+            ids=None, centres=None):
+    """FoF-seeded SOD: centre search, growth, profile and centre of mass of
+    each halo.  With ``centres`` (one particle per halo) the centre search
+    is skipped and the growth runs around the given centres."""
     pos = _pos(pos)
     seeds = np.ascontiguousarray(seeds, dtype=np.int64)
     nmem = np.ascontiguousarray(nmem, dtype=np.int64)
@@ -293,10 +299,18 @@ def sod_fof(pos, cell_start, perm, bs, seeds, nmem, params, vel=None,
     nh = len(seeds)
     out = (FofHalo * max(nh, 1))()
     prof = np.zeros((max(nh, 1), params.nbins), dtype=np.int64)
-    rc = lib().or_sod_fof(len(pos), _p(pos), _p(vel), _p(ids),
-                          _p(np.ascontiguousarray(cell_start, np.int64)),
-                          _p(np.ascontiguousarray(perm, np.int32)), bs, nh,
-                          _p(seeds), _p(nmem), C.byref(params), out, _p(prof))
+    cs = np.ascontiguousarray(cell_start, np.int64)
+    pm = np.ascontiguousarray(perm, np.int32)
+    if centres is None:
+        rc = lib().or_sod_fof(len(pos), _p(pos), _p(vel), _p(ids), _p(cs),
+                              _p(pm), bs, nh, _p(seeds), _p(nmem),
+                              C.byref(params), out, _p(prof))
+    else:
+        cen = np.ascontiguousarray(centres, dtype=np.int64)
+        assert len(cen) == nh
+        rc = lib().or_sod_fof_at(len(pos), _p(pos), _p(vel), _p(ids), _p(cs),
+                                 _p(pm), bs, nh, _p(seeds), _p(nmem),
+                                 _p(cen), C.byref(params), out, _p(prof))
     if rc != OK:
         raise ValueError(f"or_sod_fof failed: {rc}")
     res = _struct_to_dict(out, nh, FofHalo)

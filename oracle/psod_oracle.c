@@ -512,11 +512,14 @@ static inline double overdens_k(int64_t k, double mpart, double r) {
   return ((double)k * mpart) / volume;
 }
 
-int or_sod_fof(int64_t npart, const float *pos, const float *vel,
-               const int64_t *ids, const int64_t *cell_start,
-               const int32_t *perm, int32_t bs, int64_t nhalo,
-               const int64_t *seeds, const int64_t *nmem,
-               const or_fof_params *p, or_fof_halo *out, int64_t *profile) {
+/* centres == NULL: the centre search of each seed (main file:1795-1875);  */
+/* otherwise the given centre particle of each halo, and only the growth.  */
+static int sod_fof_impl(int64_t npart, const float *pos, const float *vel,
+                        const int64_t *ids, const int64_t *cell_start,
+                        const int32_t *perm, int32_t bs, int64_t nhalo,
+                        const int64_t *seeds, const int64_t *nmem,
+                        const int64_t *centres, const or_fof_params *p,
+                        or_fof_halo *out, int64_t *profile) {
   const int32_t nx = p->nx;
   const double taille = (double)nx;
   if (nx < 4 || p->ndelta < 1 || p->ndelta > OR_MAXDELTA || p->nbins < 2)
@@ -539,6 +542,13 @@ int or_sod_fof(int64_t npart, const float *pos, const float *vel,
     o->seed = s;
     o->status = OR_OK;
     if (s < 0) { o->status = OR_E_ARG; continue; }
+    int64_t c0;
+    if (centres) {
+      c0 = centres[h];
+      if (c0 < 0 || c0 >= npart) { o->status = OR_E_ARG; continue; }
+      o->robust_centre = 1;
+      goto grow;
+    }
     int32_t cs[3];
     for (int d = 0; d < 3; d++) or_cell_of(pos[3 * s + d], nx, &cs[d]);
     /* neighbours (5^3 cells) and candidates (3^3 cells), cell walk with x  */
@@ -607,7 +617,8 @@ int or_sod_fof(int64_t npart, const float *pos, const float *vel,
     o->s_best = smax;
     o->s_second = second;
     o->robust_centre = (smax - pc[best] - p->tol * smax) > worst_other;
-    const int64_t c0 = cand[best];
+    c0 = cand[best];
+  grow:
     o->centre = c0;
     o->centre_id = ids ? ids[c0] : c0;
     const double xc = XS(c0, 0), yc = XS(c0, 1), zc = XS(c0, 2);
@@ -776,6 +787,25 @@ int or_sod_fof(int64_t npart, const float *pos, const float *vel,
 #undef XS
   free(nb); free(cand); free(sc); free(pc); free(g); free(rr);
   return OR_OK;
+}
+
+int or_sod_fof(int64_t npart, const float *pos, const float *vel,
+               const int64_t *ids, const int64_t *cell_start,
+               const int32_t *perm, int32_t bs, int64_t nhalo,
+               const int64_t *seeds, const int64_t *nmem,
+               const or_fof_params *p, or_fof_halo *out, int64_t *profile) {
+  return sod_fof_impl(npart, pos, vel, ids, cell_start, perm, bs, nhalo,
+                      seeds, nmem, NULL, p, out, profile);
+}
+
+int or_sod_fof_at(int64_t npart, const float *pos, const float *vel,
+                  const int64_t *ids, const int64_t *cell_start,
+                  const int32_t *perm, int32_t bs, int64_t nhalo,
+                  const int64_t *seeds, const int64_t *nmem,
+                  const int64_t *centres, const or_fof_params *p,
+                  or_fof_halo *out, int64_t *profile) {
+  return sod_fof_impl(npart, pos, vel, ids, cell_start, perm, bs, nhalo,
+                      seeds, nmem, centres, p, out, profile);
 }
 
 /* a12: computecom (Pfof/trunk/common/src/modhalo.f90:400-432), periodic:    */

@@ -147,6 +147,16 @@ int or_sod_fof(int64_t npart, const float *pos, const float *vel,
                const int64_t *seeds, const int64_t *nmem,
                const or_fof_params *p, or_fof_halo *out, int64_t *profile);
 
+/* The growth of or_sod_fof (main file:2000-2166 and the FoF-mode outputs)
+ * around a given centre particle per halo: centres[h] replaces the centre
+ * search, whose O(n^2) cost is out of reach for halos of 10^6 members.    */
+int or_sod_fof_at(int64_t npart, const float *pos, const float *vel,
+                  const int64_t *ids, const int64_t *cell_start,
+                  const int32_t *perm, int32_t bs, int64_t nhalo,
+                  const int64_t *seeds, const int64_t *nmem,
+                  const int64_t *centres, const or_fof_params *p,
+                  or_fof_halo *out, int64_t *profile);
+
 /* ---- a12: centre of mass, Pfof/trunk/common/src/modhalo.f90:377-437 ------- */
 void or_computecom(int64_t n, const float *pos, const float *vel,
                    double *compos, double *comvel);

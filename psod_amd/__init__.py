@@ -17,8 +17,8 @@ import numpy as np
 
 from . import _lib
 from ._lib import (BLOCK, E_ARG, E_CAPACITY, E_DEVICE, E_NOMEM, E_UNCLOSED,
-                   F_VELOCITY, H_ACCEPTED, H_CLOSED, H_LARGE, H_UNCLOSED,
-                   MAXDELTA, OK, PsodError)
+                   F_CENTRE_ONLY, F_VELOCITY, H_ACCEPTED, H_CLOSED, H_LARGE,
+                   H_UNCLOSED, MAXDELTA, OK, PsodError)
 
 _L = _lib.load()
 
@@ -36,8 +36,10 @@ def _ptr(t):
 
 def sod_params(nx, npart, deltas=(200.0,), delta2=0.0, nbins=32,
                rmin_frac=0.01, rmax_frac=2.0, mmin=0.0, nmin_count=10,
-               tol_abs=None, small_max=0, velocity=False):
-    """psod_sod_params for the FoF-seeded pass (units: SOD cells)."""
+               tol_abs=None, small_max=0, velocity=False, centre_only=False):
+    """psod_sod_params for the FoF-seeded pass (units: SOD cells).
+    ``centre_only`` applies to the plain FP64 path: the centre search
+    alone."""
     p = _lib.SodParams()
     p.nx, p.ndelta = nx, len(deltas)
     for i, d in enumerate(deltas):
@@ -50,7 +52,8 @@ def sod_params(nx, npart, deltas=(200.0,), delta2=0.0, nbins=32,
     p.rsearch, p.eps = 1.0, 1.0 / 64.0
     p.tol_abs = 1e-7 * nx if tol_abs is None else tol_abs
     p.small_max = small_max
-    p.flags = F_VELOCITY if velocity else 0
+    p.flags = (F_VELOCITY if velocity else 0) | (
+        F_CENTRE_ONLY if centre_only else 0)
     return p
 
 
@@ -184,8 +187,7 @@ class Context:
 
     def seeds(self, cat, grid, nhalo, shift=0):
         """FoF seeds: densest member of each label (ties: smallest index)."""
-        key = self.torch.zeros((nhalo,), dtype=self.torch.int64,
-                               device=f"cuda:{self.device}")
+        key = self._empty((nhalo,), self.torch.int64)  # cleared by the call
         self.cic_interp(cat, grid, shift=shift, orig_out=False, nhalo=nhalo,
                         seed_key=key)
         seed = self._empty((nhalo,), self.torch.int32)
@@ -280,29 +282,42 @@ class Context:
 
     # ---- host drop-ins ---------------------------------------------------
     def host_cic(self, pos, ng, mass=None, xxmin=0.0):
+        """`cic` in FP64: grid [iz, iy, ix] of rho/rho_bar."""
         pos = np.ascontiguousarray(pos, np.float32)
         m = None if mass is None else np.ascontiguousarray(mass, np.float32)
-        grid = np.zeros((ng, ng, ng), np.float32)
+        grid = np.zeros((ng, ng, ng), np.float64)
         _lib.check(_L.psod_host_cic(self.h, len(pos), _ptr(pos), _ptr(m),
                                     float(xxmin), ng, _ptr(grid)))
         return grid
 
     def host_cicm1(self, pos, grid, xxmin=0.0):
+        """`cicm1` in FP64: density at each particle."""
         pos = np.ascontiguousarray(pos, np.float32)
-        grid = np.ascontiguousarray(grid, np.float32)
-        rho = np.zeros(len(pos), np.float32)
+        grid = np.ascontiguousarray(grid, np.float64)
+        rho = np.zeros(len(pos), np.float64)
         _lib.check(_L.psod_host_cicm1(self.h, len(pos), _ptr(pos),
                                       float(xxmin), grid.shape[0], _ptr(grid),
                                       _ptr(rho)))
         return rho
 
     def host_tri_cic(self, rho, threshold=100.0):
-        rho = np.ascontiguousarray(rho, np.float32)
+        """`tri_cic`: (isort 1-based by ascending 1/rho, npart_new)."""
+        rho = np.ascontiguousarray(rho, np.float64)
         isort = np.zeros(len(rho), np.int32)
         nn = C.c_int64(0)
         _lib.check(_L.psod_host_tri_cic(self.h, len(rho), _ptr(rho),
                                         float(threshold), _ptr(isort),
                                         C.byref(nn)))
+        return isort, nn.value
+
+    def density_order_f64(self, rho, threshold=100.0):
+        """psod_density_order_f64 on a device tensor of FP64 densities."""
+        n = rho.shape[0]
+        isort = self._empty((n,), self.torch.int32)
+        nn = C.c_int64(0)
+        self._after_torch()
+        _lib.check(_L.psod_density_order_f64(self.h, n, _ptr(rho), threshold,
+                                             _ptr(isort), C.byref(nn)))
         return isort, nn.value
 
     def host_spherover(self, pos, ids, isort, npart_new, nx, ncut=4,
@@ -352,18 +367,18 @@ def default_context():
 
 # ---- reference-named drop-ins (host arrays) --------------------------------
 def cic(pos, ng, mass=None, xxmin=0.0):
-    """`cic` (main file:1077): float32 grid [iz, iy, ix] of rho/rho_bar."""
+    """`cic` (main file:1077): FP64 grid [iz, iy, ix] of rho/rho_bar."""
     return default_context().host_cic(pos, ng, mass, xxmin)
 
 
 def cicm1(pos, grid, xxmin=0.0):
-    """`cicm1` (main file:1268): density at each particle."""
+    """`cicm1` (main file:1268): FP64 density at each particle."""
     return default_context().host_cicm1(pos, grid, xxmin)
 
 
 def tri_cic(rho, threshold=100.0):
-    """`tri_cic` (main file:998): (isort 1-based by decreasing rho,
-    npart_new)."""
+    """`tri_cic` (main file:998): (isort 1-based by ascending 1/rho, ties
+    by index, npart_new)."""
     return default_context().host_tri_cic(rho, threshold)
 
 

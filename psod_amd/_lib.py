@@ -14,6 +14,7 @@ MAXDELTA = 4
 BLOCK = 8
 H_CLOSED, H_ACCEPTED, H_LARGE, H_UNCLOSED = 1, 2, 4, 8
 F_VELOCITY = 1
+F_CENTRE_ONLY = 2
 
 EXPORTS = [
     "psod_create", "psod_destroy", "psod_last_error", "psod_synchronize",
@@ -21,7 +22,7 @@ EXPORTS = [
     "psod_cic_interp", "psod_seed_index", "psod_density_order",
     "psod_sod_fof", "psod_sod_fof_plain", "psod_label_counts",
     "psod_gen_clumps", "psod_generate", "psod_host_cic", "psod_host_cicm1",
-    "psod_host_tri_cic", "psod_host_spherover",
+    "psod_host_tri_cic", "psod_host_spherover", "psod_density_order_f64",
 ]
 
 
@@ -122,6 +123,7 @@ def load():
         "psod_host_cic": (C.c_int, [v, i64, v, v, d, i32, v]),
         "psod_host_cicm1": (C.c_int, [v, i64, v, d, i32, v, v]),
         "psod_host_tri_cic": (C.c_int, [v, i64, v, d, v, P(i64)]),
+        "psod_density_order_f64": (C.c_int, [v, i64, v, d, v, P(i64)]),
         "psod_host_spherover": (C.c_int, [v, i64, v, v, v, i64,
                                           P(GreedyParams), i64, P(i64), v, v,
                                           v, v, v, v]),

@@ -385,13 +385,14 @@ k_subsort(const int32_t *__restrict__ cs, const int32_t *__restrict__ list,
           int32_t *__restrict__ queue,
           float4 *__restrict__ rec, int32_t *__restrict__ perm,
           int32_t *__restrict__ lab, float4 *__restrict__ trec,
-          int32_t *__restrict__ tperm, int32_t *__restrict__ tlab) {
+          int32_t *__restrict__ tperm, int32_t *__restrict__ tlab,
+          int64_t tcap) {
   __shared__ int32_t hist[512];
   __shared__ int32_t item;
   const int32_t nl = min(*nlist, cap);
-  float4 *tr = trec ? trec + (int64_t)blockIdx.x * SUB_TMP : nullptr;
-  int32_t *tp = tperm ? tperm + (int64_t)blockIdx.x * SUB_TMP : nullptr;
-  int32_t *tl = tlab ? tlab + (int64_t)blockIdx.x * SUB_TMP : nullptr;
+  float4 *tr = trec ? trec + (int64_t)blockIdx.x * tcap : nullptr;
+  int32_t *tp = tperm ? tperm + (int64_t)blockIdx.x * tcap : nullptr;
+  int32_t *tl = tlab ? tlab + (int64_t)blockIdx.x * tcap : nullptr;
   for (;;) {
     if (threadIdx.x == 0) item = atomicAdd(queue, 1);
     for (int i = threadIdx.x; i < 512; i += SUB_T) hist[i] = 0;
@@ -400,7 +401,7 @@ k_subsort(const int32_t *__restrict__ cs, const int32_t *__restrict__ list,
     if (it >= nl) break;
     const int32_t c = list[it];
     const int32_t a = cs[c], n = cs[c + 1] - a;
-    if (n > SUB_TMP || (n > SUB_REG && !trec)) {
+    if (n > tcap || (n > SUB_REG && !trec)) {
       // would not fit the slice: keep the scatter order
       __syncthreads();
       continue;
@@ -572,17 +573,19 @@ extern "C" int psod_cell_sort(psod_ctx *ctx, int64_t npart, const float *d_pos,
     // larger cells: one workgroup per CU, each with a staging slice
     k_subsort<<<(unsigned)ctx->num_cus * 8, SUB_T, 0, st>>>(
         d_cell_start, list, ctl, cap, ctl + 1, (float4 *)d_rec, d_perm,
-        d_lab_sorted, nullptr, nullptr, nullptr);
+        d_lab_sorted, nullptr, nullptr, nullptr, 0);
     PSOD_LAUNCH_CHECK();
-    const unsigned nsub = (unsigned)ctx->num_cus;
-    char *tmp = (char *)psod_scratch(ctx, S_KEYS,
-                                     (size_t)nsub * SUB_TMP * 24);
+    // cells above SUB_REG records: at most npart / SUB_REG of them, each
+    // staged in a slice of at most min(SUB_TMP, npart) records
+    const int64_t tcap = std::min<int64_t>(SUB_TMP, npart);
+    const unsigned nsub = (unsigned)std::min<int64_t>(ctx->num_cus,
+                                                      npart / SUB_REG + 1);
+    char *tmp = (char *)psod_scratch(ctx, S_KEYS, (size_t)nsub * tcap * 24);
     if (!tmp) return PSOD_E_NOMEM;
     k_subsort<<<nsub, SUB_T, 0, st>>>(
         d_cell_start, big, ctl + 2, cap, ctl + 3, (float4 *)d_rec, d_perm,
-        d_lab_sorted, (float4 *)tmp,
-        (int32_t *)(tmp + (size_t)nsub * SUB_TMP * 16),
-        (int32_t *)(tmp + (size_t)nsub * SUB_TMP * 20));
+        d_lab_sorted, (float4 *)tmp, (int32_t *)(tmp + (size_t)nsub * tcap * 16),
+        (int32_t *)(tmp + (size_t)nsub * tcap * 20), tcap);
     PSOD_LAUNCH_CHECK();
   }
   PSOD_HIP(hipStreamSynchronize(st));

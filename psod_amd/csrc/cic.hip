@@ -451,6 +451,10 @@ extern "C" int psod_cic_interp(psod_ctx *ctx, int64_t npart, int32_t nx,
     return psod_set_error(PSOD_E_ARG, "psod_cic_interp: bad arguments");
   const int32_t nb = nx / 8;
   const unsigned nblk = (unsigned)nb * nb * nb;
+  // the keys are maxima over the members: clear them on this stream, so
+  // that no fill queued elsewhere can race the atomics below
+  if (d_seed_key && d_lab_sorted && nhalo > 0)
+    PSOD_TRY(psod_fill_u64(ctx, d_seed_key, nhalo, 0));
   if (ng == nx)
     k_cic_interp<1><<<nblk, 256, 0, ctx->stream>>>(
         d_cell_start, (const float4 *)d_rec, nx, ng, shift, d_grid,

@@ -431,6 +431,13 @@ extern "C" int psod_host_spherover(psod_ctx *ctx, int64_t npart,
     return psod_set_error(PSOD_E_ARG, "spherover: bad parameters");
   *nrec = 0;
   if (npart_new > npart) npart_new = npart;
+  // the seeds index the device's particle arrays: reject any outside them
+  // before anything is queued
+  for (int64_t o = 0; o < npart_new; o++)
+    if (isort[o] < 1 || isort[o] > npart)
+      return psod_set_error(PSOD_E_ARG,
+                            "spherover: isort(%lld) = %d outside [1, %lld]",
+                            (long long)(o + 1), isort[o], (long long)npart);
   GreedyCfg c;
   c.nx = p->nx;
   c.ncut = p->ncut;

@@ -9,8 +9,6 @@
 // call.
 #include "psod_internal.h"
 
-int psod_fill_u64(psod_ctx *c, uint64_t *p, int64_t n, uint64_t v);
-
 extern "C" int psod_pass(psod_ctx *ctx, const psod_sod_params *p,
                          int64_t npart, int32_t nx, int32_t ng,
                          const float *d_pos, const float *d_vel,
@@ -45,7 +43,6 @@ extern "C" int psod_pass(psod_ctx *ctx, const psod_sod_params *p,
                           labs));
   PSOD_TRY(psod_cic_deposit(ctx, npart, nx, cs, rec, ng, 0, grid));
   if (nhalo == 0) return PSOD_OK;
-  PSOD_TRY(psod_fill_u64(ctx, keys, nhalo, 0));
   PSOD_TRY(psod_cic_interp(ctx, npart, nx, cs, rec, ng, 0, grid, nullptr,
                            nullptr, labs, nhalo, keys));
   PSOD_TRY(psod_seed_index(ctx, nhalo, keys, seed));

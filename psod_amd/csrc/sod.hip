@@ -358,9 +358,10 @@ k_sod_small(SodCfg c, const int32_t *__restrict__ cell_start,
       const int j = have ? S.a.cidx[cb + l] : 0;
       const float4 p = S.a.nb[j];
       uint64_t acc = 0;
-      // u32 partial sums of at most 256 terms of at most 2^24
-      for (int32_t t0 = 0; t0 < n5u; t0 += 256) {
-        const int32_t t1 = min(n5u, t0 + 256);
+      // u32 partial sums of at most 255 terms of at most 2^24 (256 terms
+      // at d = 0, duplicated particles, would reach 2^32)
+      for (int32_t t0 = 0; t0 < n5u; t0 += 255) {
+        const int32_t t1 = min(n5u, t0 + 255);
         uint32_t part = 0;
 #pragma unroll 4
         for (int32_t t = t0; t < t1; t++) {
@@ -2442,11 +2443,10 @@ static int centre_search(psod_ctx *ctx, const SodCfg &c,
   return PSOD_OK;
 }
 
-extern "C" int psod_sod_fof(psod_ctx *ctx, const psod_sod_params *p,
-                            const psod_catalogue *cat, int64_t nhalo,
-                            const int32_t *d_seed, const int32_t *d_nmem,
-                            const psod_halo_out *out) {
-  PSOD_TRY(psod_use_device(ctx));
+static int sod_fof_run(psod_ctx *ctx, const psod_sod_params *p,
+                       const psod_catalogue *cat, int64_t nhalo,
+                       const int32_t *d_seed, const int32_t *d_nmem,
+                       const psod_halo_out *out) {
   PSOD_TRY(check_sod_args(p, cat, nhalo, d_seed, out));
   if (nhalo <= 0) return PSOD_OK;
   if (!out->profile)
@@ -2618,4 +2618,20 @@ extern "C" int psod_sod_fof(psod_ctx *ctx, const psod_sod_params *p,
                           "sod: %d halos do not close within half the box",
                           hf[1]);
   return PSOD_OK;
+}
+
+// Every return, failed ones included, waits for both streams: the small
+// halos' kernel may still be writing the caller's outputs when a later step
+// fails, and the caller may free them as soon as this returns.
+extern "C" int psod_sod_fof(psod_ctx *ctx, const psod_sod_params *p,
+                            const psod_catalogue *cat, int64_t nhalo,
+                            const int32_t *d_seed, const int32_t *d_nmem,
+                            const psod_halo_out *out) {
+  PSOD_TRY(psod_use_device(ctx));
+  const int rc = sod_fof_run(ctx, p, cat, nhalo, d_seed, d_nmem, out);
+  if (rc != PSOD_OK) {
+    (void)hipStreamSynchronize(ctx->side);
+    (void)hipStreamSynchronize(ctx->stream);
+  }
+  return rc;
 }

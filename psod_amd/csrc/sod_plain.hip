@@ -213,6 +213,12 @@ k_plain_grow(SodCfg c, const PlainHalo *__restrict__ ph, int64_t h,
   const int32_t centre = (int32_t)(bk & 0xFFFFFFFF);
   res.centre = centre;
   res.centre_id = ids ? ids[centre] : (int64_t)centre;
+  if (c.flags & PSOD_F_CENTRE_ONLY) {
+    if (threadIdx.x == 0) store_result(out, h, c.nbins, res);
+    for (int j = threadIdx.x; j < c.nbins; j += PT)
+      out.profile[h * c.nbins + j] = 0;
+    return;
+  }
   const CellRef cr = ref_of_particle(pos, centre, c.nx);
   double xcs[3];
   centre_scaled(pos, centre, c.nx, xcs);

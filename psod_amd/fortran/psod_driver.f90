@@ -5,10 +5,12 @@
 ! Input (-inp FILE): raw little-endian catalogue
 !   int64 npart, real(4) posf(3,npart) in [0,1), int64 idf(npart).
 ! Flags as the reference (main file:811-852): -nx, -min, -del, -dl2, -sel,
-! -nse, -nch.  Output: one line per accepted halo (see spherover).
+! -nse, -nch.  Output: one line per accepted halo (see spherover); -rec FILE
+! also writes every grown sphere's record to FILE at full precision.
 program psod_driver
   use modvar
   use modcic
+  use psod_capi, only: psod_record_file
   implicit none
   character(len=256) :: inp, opt, arg
   character(len=5) :: nchar
@@ -41,13 +43,14 @@ program psod_driver
     case ('-del'); read(arg, *) seuil
     case ('-dl2'); read(arg, *) seuil2
     case ('-nch'); nchar = arg(1:5)
+    case ('-rec'); psod_record_file = arg
     case default; write(*, *) 'unknown option ', trim(opt), ' ignored'
     end select
     i = i + 2
   end do
   if (nx <= 0 .or. len_trim(inp) == 0) then
     write(*, *) 'usage: psod_driver -inp FILE -nx NX [-min 100 -del 200 ', &
-        '-dl2 800 -sel 4 -nse 4000000 -nch 00001]'
+        '-dl2 800 -sel 4 -nse 4000000 -nch 00001 -rec FILE]'
     stop 2
   end if
 

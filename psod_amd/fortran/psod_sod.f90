@@ -28,6 +28,9 @@ module psod_capi
   use iso_c_binding
   implicit none
   type(c_ptr), save :: psod_handle = c_null_ptr
+  ! when not blank, spherover also writes its records to this file (no
+  ! reference counterpart: a full-precision copy of the halo lines)
+  character(len=512), save :: psod_record_file = ''
 
   type, bind(C) :: psod_greedy_params
     integer(c_int32_t) :: nx, ncut
@@ -54,24 +57,25 @@ module psod_capi
       real(c_float), intent(in) :: pos(*), mass(*)
       real(c_double), value :: xxmin
       integer(c_int32_t), value :: ng
-      real(c_float), intent(out) :: grid(*)
+      real(c_double), intent(out) :: grid(*)
     end function
     integer(c_int) function psod_host_cicm1(ctx, npart, pos, xxmin, ng, grid, &
         rho) bind(C, name="psod_host_cicm1")
       import :: c_int, c_ptr, c_int64_t, c_int32_t, c_float, c_double
       type(c_ptr), value :: ctx
       integer(c_int64_t), value :: npart
-      real(c_float), intent(in) :: pos(*), grid(*)
+      real(c_float), intent(in) :: pos(*)
+      real(c_double), intent(in) :: grid(*)
       real(c_double), value :: xxmin
       integer(c_int32_t), value :: ng
-      real(c_float), intent(out) :: rho(*)
+      real(c_double), intent(out) :: rho(*)
     end function
     integer(c_int) function psod_host_tri_cic(ctx, npart, rho, threshold, &
         isort, npart_new) bind(C, name="psod_host_tri_cic")
       import :: c_int, c_ptr, c_int64_t, c_int32_t, c_float, c_double
       type(c_ptr), value :: ctx
       integer(c_int64_t), value :: npart
-      real(c_float), intent(in) :: rho(*)
+      real(c_double), intent(in) :: rho(*)
       real(c_double), value :: threshold
       integer(c_int32_t), intent(out) :: isort(*)
       integer(c_int64_t), intent(out) :: npart_new
@@ -106,8 +110,8 @@ contains
     if (rc /= 0) error stop 'psod: no device'
   end subroutine
 
-  ! the reference's error convention: print the message and stop the zone
-  subroutine psod_check(rc, what)
+  ! prints the library's message for a non-zero status
+  subroutine psod_report(rc, what)
     integer(c_int), intent(in) :: rc
     character(len=*), intent(in) :: what
     character(kind=c_char), pointer :: msg(:)
@@ -120,6 +124,16 @@ contains
       n = n + 1
     end do
     write(*, '(a,a,i0,a,1024a)') what, ': status ', rc, ': ', msg(1:n)
+  end subroutine
+
+  ! a failed cic, cicm1 or tri_cic leaves cube, rho or isort unusable, and
+  ! every later step of the zone would read them: print and stop
+  subroutine psod_check(rc, what)
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: what
+    if (rc == 0) return
+    call psod_report(rc, what)
+    error stop 'psod: the zone cannot continue'
   end subroutine
 
 end module psod_capi
@@ -138,21 +152,20 @@ subroutine cic(npart, xxmin, xxmax, yymin, yymax, zzmin, zzmax, xmin, xmax, &
   real(8) :: xxmin, xxmax, yymin, yymax, zzmin, zzmax
   real(8) :: xmin, xmax, ymin, ymax, zmin, zmax
   logical :: periodic
-  real(c_float), allocatable :: grid(:,:,:)
+  real(c_double), allocatable :: grid(:,:,:)
   integer(c_int) :: rc
   call psod_init()
-  allocate(cube(0:local_nx, 0:local_ny, 0:local_nz))
-  cube = 0.d0
   if (.not. periodic .or. nx /= ny .or. nx /= nz .or. local_nx < nx - 1) then
     write(*, *) 'psod cic: only one periodic zone covering the box'
-    return
+    error stop 'psod: unsupported zone layout'
   end if
+  allocate(cube(0:local_nx, 0:local_ny, 0:local_nz))
+  cube = 0.d0
   allocate(grid(0:nx-1, 0:ny-1, 0:nz-1))
   rc = psod_host_cic(psod_handle, int(npart, c_int64_t), posf, mpf, &
                      real(xxmin, c_double), int(nx, c_int32_t), grid)
   call psod_check(rc, 'cic')
-  if (rc /= 0) return
-  cube(0:nx-1, 0:ny-1, 0:nz-1) = real(grid, 8)
+  cube(0:nx-1, 0:ny-1, 0:nz-1) = grid
   deallocate(grid)
 end subroutine cic
 
@@ -167,18 +180,16 @@ subroutine cicm1(npart, xxmin, xxmax, yymin, yymax, zzmin, zzmax, local_nx, &
   integer :: npart, local_nx, local_ny, local_nz, nx, ny, nz
   real(8) :: xxmin, xxmax, yymin, yymax, zzmin, zzmax
   logical :: periodic
-  real(c_float), allocatable :: grid(:,:,:), r(:)
+  real(c_double), allocatable :: grid(:,:,:)
   integer(c_int) :: rc
   call psod_init()
   allocate(rho(npart))
-  rho = 0.d0
-  allocate(grid(0:nx-1, 0:ny-1, 0:nz-1), r(npart))
-  grid = real(cube(0:nx-1, 0:ny-1, 0:nz-1), c_float)
+  allocate(grid(0:nx-1, 0:ny-1, 0:nz-1))
+  grid = cube(0:nx-1, 0:ny-1, 0:nz-1)
   rc = psod_host_cicm1(psod_handle, int(npart, c_int64_t), posf, &
-                       real(xxmin, c_double), int(nx, c_int32_t), grid, r)
+                       real(xxmin, c_double), int(nx, c_int32_t), grid, rho)
   call psod_check(rc, 'cicm1')
-  if (rc == 0) rho = real(r, 8)
-  deallocate(grid, r)
+  deallocate(grid)
 end subroutine cicm1
 
 ! tri_cic (main file:998-1006): pdens = sorted 1/rho, isort, npart_new;
@@ -192,21 +203,18 @@ subroutine tri_cic(pdens, isort, npart, nx, ny, nz, npart_new, seuil)
   integer, dimension(1:npart) :: isort
   real(8), dimension(1:npart) :: pdens
   real(8) :: seuil
-  real(c_float), allocatable :: r(:)
   integer(c_int64_t) :: nn
   integer(c_int) :: rc
   integer :: i
   call psod_init()
-  allocate(r(npart))
-  r = real(rho, c_float)
-  rc = psod_host_tri_cic(psod_handle, int(npart, c_int64_t), r, &
+  rc = psod_host_tri_cic(psod_handle, int(npart, c_int64_t), rho, &
                          100.0_c_double, isort, nn)
   call psod_check(rc, 'tri_cic')
   npart_new = int(nn)
   do i = 1, npart
     pdens(i) = 1.d0 / rho(isort(i))
   end do
-  deallocate(rho, r)
+  deallocate(rho)
 end subroutine tri_cic
 
 ! spherover (main file:1396-1400): greedy SOD; positions in posf are already
@@ -238,7 +246,7 @@ subroutine spherover(nchar, npart, nx, ny, nz, npart_new, ncut, seuil, &
   integer(c_int8_t), allocatable :: alive(:)
   integer(c_int64_t) :: nrec, maxrec
   integer(c_int) :: rc
-  integer :: k, nacc
+  integer :: k, nacc, u
   call psod_init()
   p%nx = nx
   p%ncut = ncut
@@ -256,8 +264,15 @@ subroutine spherover(nchar, npart, nx, ny, nz, npart_new, ncut, seuil, &
   rc = psod_host_spherover(psod_handle, int(npart, c_int64_t), pos, idf, &
                            isort, int(npart_new, c_int64_t), p, maxrec, nrec, &
                            ri, rf, rb, mb, alive, structure)
-  call psod_check(rc, 'spherover')
-  if (rc /= 0) return
+  ! the reference drops the rest of the zone with a message when the gather
+  ! overflows Nsel or a sphere does not close (main file:1681-1693,
+  ! 2072-2089), after the halos of the earlier seeds were written; any other
+  ! failure stops
+  if (rc == 3 .or. rc == 4) then
+    call psod_report(rc, 'spherover')
+  else
+    call psod_check(rc, 'spherover')
+  end if
   nacc = 0
   do k = 1, int(nrec)
     if (rf(8, k) > 0.5d0) then
@@ -272,5 +287,18 @@ subroutine spherover(nchar, npart, nx, ny, nz, npart_new, ncut, seuil, &
   end do
   write(*, '(a,i0,a,i0,a)') 'psod spherover: ', nacc, ' halos accepted of ', &
       nrec, ' grown spheres'
+  ! every grown sphere, full precision, when the caller named a file:
+  ! int64 nrec, then per sphere int64 (ipart, seed, centre, iamas, masse,
+  ! masse2) and real(8) (amass, xc, yc, zc, rayon, overdens, overdens2,
+  ! accepted), positions in cells
+  if (len_trim(psod_record_file) > 0) then
+    open(newunit=u, file=trim(psod_record_file), access='stream', &
+         form='unformatted', status='replace')
+    write(u) nrec
+    do k = 1, int(nrec)
+      write(u) ri(:, k), rf(:, k)
+    end do
+    close(u)
+  end if
   deallocate(pos, ri, rf, rb, mb, alive, structure)
 end subroutine spherover

@@ -1,0 +1,236 @@
+"""Parity at the benchmark workloads' full sizes: 512^3 (60,000 clumps),
+1024^3 (500,000 clumps, one or four thresholds) and the zoom-like 1024^3
+catalogue with a 5e7-particle central halo, each run through the batched
+pass on the whole box and checked against the FP64 oracle.
+
+The oracle cannot run a 10^9-particle box, so each check takes sub-boxes
+(tests/parity.py: exact for power-of-two nx) and compares every halo seeded
+well inside one.  The largest halos get their own sub-box: their centre is
+checked against the plain FP64 path's brute-force centre search on the
+same catalogue (a different centre is accepted only when both have the same
+FP64 S to 1e-5, recomputed here), and their growth, profile, second
+threshold and centre of mass against the oracle's growth around that
+centre.  The CIC grid is checked node by node on a sub-box interior.
+"""
+import gc
+
+import numpy as np
+import pytest
+
+from parity import (check_sod, com_to_full, host, oracle_subbox,
+                    reach_inside, s_fp64, subbox)
+
+pytestmark = pytest.mark.gpu
+
+
+def log(*a):
+    print(*a, flush=True)
+
+
+def workload(ctx, nx, nclump, nhi, seed=1, ncentral=0, with_vel=False,
+             deltas=(200.0,), nbins=32, rmin=0.01, rmax=2.0):
+    """Generates the catalogue on the device and runs the batched pass."""
+    import psod_amd as P
+    N = nx ** 3
+    g = P.gen_params(N, nclump, nlo=20, nhi=nhi, seed=seed,
+                     ncentral=ncentral, ccentral=6.0)
+    pos, vel, lab, ids, tab = ctx.generate(g, with_vel=with_vel)
+    del ids
+    nh = len(tab["size"])
+    sod_kw = dict(deltas=deltas, delta2=800.0, nbins=nbins, rmin_frac=rmin,
+                  rmax_frac=rmax, mmin=100.0)
+    p = P.sod_params(nx, N, velocity=with_vel, **sod_kw)
+    cat = ctx.cell_sort(pos, nx, labels=lab, want_perm=False)
+    grid = ctx.cic_deposit(cat, nx)
+    seeds = ctx.seeds(cat, grid, nh)
+    nmem = ctx.label_counts(lab, nh)
+    o = ctx.sod_fof(p, cat, pos, seeds, nmem, vel=vel)
+    o = {k: host(v) for k, v in o.items()}
+    log(f"nx={nx}: {nh} halos, status closed {(o['status'] & 1).sum()}")
+    return dict(nx=nx, N=N, nh=nh, pos=pos, vel=vel, lab=lab, cat=cat,
+                grid=grid, seeds=seeds, nmem=nmem, o=o, p=p, sod_kw=sod_kw,
+                seeds_h=host(seeds), nmem_h=host(nmem).astype(np.int64))
+
+
+def release(w):
+    w.clear()
+    gc.collect()
+    import torch
+    torch.cuda.empty_cache()
+
+
+def sel(o, idx):
+    return {k: v[idx] for k, v in o.items()}
+
+
+def check_subbox(orc, w, lo, sub, margin, max_halos=400, max_nmem=20000,
+                 check_cic=True):
+    """Every halo (up to max_halos, members <= max_nmem) seeded at least
+    `margin` cells inside the sub-box, against the oracle; the CIC nodes of
+    the sub-box interior against the oracle's CIC.  Returns the number of
+    halos compared."""
+    nx = w["nx"]
+    spos, sidx, svel = subbox(w["pos"], nx, lo, sub, w["vel"])
+    sd = w["seeds_h"]
+    ok = sd >= 0
+    sc = np.floor(host(w["pos"][w["seeds"].clamp(min=0).long()]) * nx)
+    inner = ok & np.all((sc >= np.array(lo) + margin) &
+                        (sc < np.array(lo) + sub - margin), axis=1)
+    inner &= w["nmem_h"] <= max_nmem
+    hs = np.where(inner)[0]
+    if len(hs) > max_halos:
+        hs = np.sort(np.random.default_rng(0).choice(hs, max_halos,
+                                                     replace=False))
+    log(f"sub-box {lo}+{sub}: {len(spos)} particles, {len(hs)} halos")
+    mpart = float(nx) ** 3 / float(w["N"])
+    r = oracle_subbox(orc, spos, sidx, sub, sd[hs], w["nmem_h"][hs], mpart,
+                      w["sod_kw"], vel=svel)
+    assert np.all(r["status"] == orc.OK)
+    keep = reach_inside(r, spos, sub)
+    assert keep.mean() > 0.95, keep.mean()
+    r = sel(r, keep)
+    hs = hs[keep]
+    r["com_pos"] = com_to_full(r["com_pos"], lo, sub, nx)
+    o = sel(w["o"], hs)
+    nd = len(w["sod_kw"]["deltas"])
+    same = check_sod(o, r, nd)
+    if check_cic:
+        # nodes 1..sub-1 of the sub-box get mass only from its own cells
+        m = np.full(len(spos), np.float32(1.0 / w["N"]), np.float32)
+        og = orc.cic(spos, sub, mass=m) * (nx / sub) ** 3
+        gg = host(w["grid"][lo[2] + 1:lo[2] + sub, lo[1] + 1:lo[1] + sub,
+                            lo[0] + 1:lo[0] + sub]).astype(np.float64)
+        og = og[1:, 1:, 1:]
+        err = np.abs(gg - og)
+        assert np.all(err <= 1e-6 * np.abs(og) + 1e-9), err.max()
+        nz = og > 0
+        log(f"  CIC nodes {og.size}: max rel err "
+            f"{(err[nz] / og[nz]).max():.3g}")
+    rel_r = np.abs(o["rint"][same, 0] / r["rint"][same, 0] - 1).max()
+    log(f"  {len(hs)} halos, {same.sum()} same centre; max rel err "
+        f"R_int {rel_r:.3g}")
+    return len(hs)
+
+
+def check_giants(ctx, orc, w, k=5, min_nmem=0):
+    """The k largest halos: centre against the plain FP64 centre search,
+    growth against the oracle's growth around that centre."""
+    import psod_amd as P
+    import torch
+    nx, o = w["nx"], w["o"]
+    order = np.argsort(-w["nmem_h"], kind="stable")
+    cand = [h for h in order[:4 * k] if w["seeds_h"][h] >= 0]
+    done = 0
+    for h in cand:
+        if done == k:
+            break
+        if w["nmem_h"][h] < min_nmem:
+            continue
+        W = int(o["w"][h])
+        sub = 8 * ((2 * W + 16 + 7) // 8)
+        cc = np.floor(host(w["pos"][int(o["centre"][h])]) * nx).astype(int)
+        lo = [int(min(max(c - sub // 2, 0), nx - sub)) for c in cc]
+        if any(c - W - 2 < l or c + W + 2 >= l + sub for c, l in zip(cc, lo)):
+            log(f"giant {h}: reach {W} does not fit a sub-box, skipped")
+            continue
+        # FP64 brute-force centre on the device (plain path, centre only)
+        pp = P.sod_params(nx, w["N"], centre_only=True, **w["sod_kw"])
+        sd = torch.tensor([int(w["seeds_h"][h])], dtype=torch.int32,
+                          device="cuda:0")
+        nm = torch.tensor([int(w["nmem_h"][h])], dtype=torch.int32,
+                          device="cuda:0")
+        op = ctx.sod_fof(pp, w["cat"], w["pos"], sd, nm, plain=True)
+        cpl = int(host(op["centre"])[0])
+        spos, sidx, svel = subbox(w["pos"], nx, lo, sub, w["vel"])
+        cb = int(o["centre"][h])
+        if cpl != cb:
+            s = s_fp64(spos, sub, np.searchsorted(sidx, [cb, cpl]))
+            log(f"giant {h}: centres {cb} (batched) {cpl} (plain FP64), "
+                f"S {s[0]:.10g} {s[1]:.10g}")
+            assert abs(s[0] - s[1]) <= 1e-5 * s[1]
+        mpart = float(nx) ** 3 / float(w["N"])
+        r = oracle_subbox(orc, spos, sidx, sub, w["seeds_h"][[h]],
+                          w["nmem_h"][[h]], mpart, w["sod_kw"], vel=svel,
+                          centres=np.array([cb]))
+        assert r["status"][0] == orc.OK
+        assert reach_inside(r, spos, sub)[0]
+        r["com_pos"] = com_to_full(r["com_pos"], lo, sub, nx)
+        check_sod(sel(o, [h]), r, len(w["sod_kw"]["deltas"]))
+        log(f"giant {h}: {w['nmem_h'][h]} members, reach {W}, sub-box "
+            f"{len(spos)} particles, count {o['count'][h, 0]} "
+            f"(oracle {r['count'][0, 0]}), R {o['radius'][h, 0]:.6f}")
+        done += 1
+    return done
+
+
+def test_c2_512(ctx, orc):
+    w = workload(ctx, 512, 60000, 2e5)
+    try:
+        n = check_subbox(orc, w, (64, 64, 64), 128, 20)
+        n += check_subbox(orc, w, (320, 192, 256), 128, 20)
+        assert n > 200
+        assert check_giants(ctx, orc, w, k=3) == 3
+    finally:
+        release(w)
+
+
+def test_c3_1024(ctx, orc):
+    w = workload(ctx, 1024, 500000, 2e6)
+    try:
+        n = check_subbox(orc, w, (128, 640, 384), 128, 24)
+        n += check_subbox(orc, w, (768, 128, 512), 128, 24, check_cic=False)
+        assert n > 200
+        assert check_giants(ctx, orc, w, k=5) == 5
+    finally:
+        release(w)
+
+
+def test_c4_zoom_1024(ctx, orc):
+    """The central 5e7-particle halo (centre against the plain FP64 centre
+    search, growth against the oracle) and satellites away from it."""
+    w = workload(ctx, 1024, 200000, 1e5, ncentral=50_000_000)
+    try:
+        h = int(np.argmax(w["nmem_h"]))
+        assert w["nmem_h"][h] == 50_000_000
+        assert check_giants(ctx, orc, w, k=1) == 1
+        n = check_subbox(orc, w, (32, 32, 32), 128, 20, check_cic=False)
+        assert n > 30
+    finally:
+        release(w)
+
+
+def test_c4_reduced_plain_and_oracle(ctx, orc):
+    """256^3 with a 2e6-particle central clump: the batched result of the
+    central halo equals the plain FP64 path's on every output, and the
+    satellites match the oracle."""
+    w = workload(ctx, 256, 3000, 1e5, ncentral=2_000_000)
+    try:
+        h = int(np.argmax(w["nmem_h"]))
+        sd = w["seeds"][h:h + 1].contiguous()
+        nm = w["nmem"][h:h + 1].contiguous()
+        op = ctx.sod_fof(w["p"], w["cat"], w["pos"], sd, nm, plain=True)
+        op = {k: host(v) for k, v in op.items()}
+        ob = sel(w["o"], [h])
+        assert op["centre"][0] == ob["centre"][0]
+        for k in ("count", "count2", "profile", "status"):
+            np.testing.assert_array_equal(ob[k], op[k], err_msg=k)
+        np.testing.assert_array_equal(ob["radius"], op["radius"])
+        np.testing.assert_allclose(ob["rint"], op["rint"], rtol=1e-12)
+        assert check_giants(ctx, orc, w, k=1) == 1
+        n = check_subbox(orc, w, (16, 16, 16), 96, 16)
+        assert n > 20
+    finally:
+        release(w)
+
+
+def test_c5_four_thresholds_1024(ctx, orc):
+    """Four thresholds, 64 bins on [0.005, 3] R, centre-of-mass velocities."""
+    w = workload(ctx, 1024, 500000, 2e6, with_vel=True,
+                 deltas=(200.0, 500.0, 740.74, 359.3), nbins=64, rmin=0.005,
+                 rmax=3.0)
+    try:
+        n = check_subbox(orc, w, (256, 512, 768), 128, 28, check_cic=False)
+        assert n > 100
+        assert check_giants(ctx, orc, w, k=2) == 2
+    finally:
+        release(w)

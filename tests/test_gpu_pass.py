@@ -10,9 +10,10 @@ import os
 import numpy as np
 import pytest
 
+from parity import check_sod
+
 pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
-TOL_TIE = 1e-5
 
 
 def catalogue(ctx, nx, nclump, nhi, seed, with_vel=True):
@@ -148,7 +149,7 @@ def test_seeds_and_density_order(ctx, orc, c64):
 
 
 def sod_compare(ctx, orc, cat_d, deltas=(200.0,), delta2=800.0, nbins=32,
-                rmax=2.0, plain=False):
+                rmin=0.01, rmax=2.0, plain=False):
     import psod_amd as P
     nx, N = cat_d["nx"], cat_d["N"]
     cat = ctx.cell_sort(cat_d["pos"], nx, labels=cat_d["lab"])
@@ -156,54 +157,27 @@ def sod_compare(ctx, orc, cat_d, deltas=(200.0,), delta2=800.0, nbins=32,
     seeds = ctx.seeds(cat, grid, cat_d["nh"])
     nmem = ctx.label_counts(cat_d["lab"], cat_d["nh"])
     p = P.sod_params(nx, N, deltas=deltas, delta2=delta2, nbins=nbins,
-                     rmax_frac=rmax, velocity=True)
+                     rmin_frac=rmin, rmax_frac=rmax, mmin=100.0,
+                     velocity=True)
     o = ctx.sod_fof(p, cat, cat_d["pos"], seeds, nmem, vel=cat_d["vel"],
                     ids=cat_d["ids"], plain=plain)
     o = {k: host(v) for k, v in o.items()}
     hp = host(cat_d["pos"])
     cs, perm = orc.cell_sort(hp, nx, 8)
     fp = orc.fof_params(nx, N, deltas=deltas, delta2=delta2, nbins=nbins,
-                        rmax_frac=rmax)
+                        rmin_frac=rmin, rmax_frac=rmax, mmin=100.0)
     r = orc.sod_fof(hp, cs, perm, 8, host(seeds).astype(np.int64),
                     host(nmem).astype(np.int64), fp, vel=host(cat_d["vel"]),
                     ids=host(cat_d["ids"]))
     return o, r
 
 
-def check_sod(o, r, nd):
-    assert np.all(o["status"] & 1), "every halo closes"
-    same = o["centre"] == r["centre"]
-    robust = r["robust_centre"] == 1
-    assert np.all(same | ~robust), np.where(~same & robust)
-    assert same.mean() > 0.99
-    np.testing.assert_array_equal(o["centre_id"][same], r["centre_id"][same])
-    for d in range(nd):
-        cnt = o["count"][:, d] == r["count"][:, d]
-        rob = r["margin"][:, d] > TOL_TIE
-        assert np.all(cnt | ~same | ~rob)
-        # discrete R is the FP64 distance of the same particle
-        ok = same & cnt
-        np.testing.assert_allclose(o["radius"][ok, d], r["radius"][ok, d],
-                                   rtol=1e-12)
-        # interpolated R and M are continuous across near-ties
-        np.testing.assert_allclose(o["rint"][same, d], r["rint"][same, d],
-                                   rtol=1e-6)
-        np.testing.assert_allclose(o["mint"][same, d], r["mint"][same, d],
-                                   rtol=3e-6)
-    ok = same & (o["count"][:, 0] == r["count"][:, 0])
-    c2 = o["count2"] == r["count2"]
-    assert np.all(c2 | ~ok | ~(r["margin2"] > TOL_TIE))
-    pm = (o["profile"] == r["profile"]).all(axis=1)
-    assert np.all(pm | ~ok | ~(r["profile_margin"] > TOL_TIE))
-    np.testing.assert_allclose(o["com_vel"][ok], r["com_vel"][ok], rtol=1e-9,
-                               atol=1e-9)
-    dp = np.abs(o["com_pos"][ok] - r["com_pos"][ok])
-    assert np.all(np.minimum(dp, 1 - dp) < 1e-12)
-
-
 def test_sod_fof_multi_delta(ctx, orc, c64):
+    """The four thresholds, 64 bins on [0.005, 3] R of the multi-threshold
+    workload."""
     deltas = (200.0, 500.0, 740.74, 359.3)
-    o, r = sod_compare(ctx, orc, c64, deltas=deltas, nbins=64, rmax=3.0)
+    o, r = sod_compare(ctx, orc, c64, deltas=deltas, nbins=64, rmin=0.005,
+                       rmax=3.0)
     check_sod(o, r, 4)
 
 
@@ -337,20 +311,45 @@ def test_empty_label_and_errors(ctx, c64):
         ctx.sod_fof(bad, cat, c64["pos"], seeds)
 
 
-def test_host_dropins_reference_grid(ctx, orc):
+def same_order_up_to_ties(isort, oisort, key):
+    """isort equals oisort except inside runs of exactly equal keys, where
+    the reference's quick_sort permutes ties its own way."""
+    a = key[isort - 1]
+    np.testing.assert_array_equal(a, key[oisort - 1])
+    diff = np.where(isort != oisort)[0]
+    for i in diff:
+        assert a[i] == a[i - 1] or (i + 1 < len(a) and a[i] == a[i + 1])
+
+
+@pytest.mark.parametrize("nx,nclump,xxmin", [(32, 40, -0.5), (32, 40, -0.3),
+                                             (128, 2000, -0.5)])
+def test_host_dropins_fp64(ctx, orc, nx, nclump, xxmin):
+    """cic, cicm1 and tri_cic of the drop-in in FP64 against the oracle:
+    the deposit differs only by the order of its FP64 additions, the
+    interpolation of a given grid is the oracle's to the bit, and the
+    density order of the oracle's densities is the oracle's order."""
     import psod_amd as P
-    nx = 32
-    opos = orc.generate(orc.gen_params(nx ** 3, 40, nlo=20, nhi=3000,
+    opos = orc.generate(orc.gen_params(nx ** 3, nclump, nlo=20, nhi=20000,
                                        seed=5), with_vel=False)[0]
-    g = P.cic(opos, 2 * nx, xxmin=-0.5)
-    og = orc.cic(opos, 2 * nx, xxmin=-0.5)
-    assert np.all(np.abs(g - og) <= 1e-6 * np.abs(og) + 1e-9)
-    rh = P.cicm1(opos, g, xxmin=-0.5)
-    np.testing.assert_allclose(rh, orc.cicm1(opos, og, xxmin=-0.5), rtol=1e-6)
-    isort, nn = P.tri_cic(rh)
-    assert nn == int((rh > 100).sum()) + 1
-    np.testing.assert_array_equal(isort - 1,
-                                  np.lexsort((np.arange(len(rh)), -rh)))
+    g = P.cic(opos, 2 * nx, xxmin=xxmin)
+    og = orc.cic(opos, 2 * nx, xxmin=xxmin)
+    assert g.dtype == np.float64
+    # each node sums at most a few thousand terms: |err| < 1e-12 relative
+    np.testing.assert_allclose(g, og, rtol=1e-12, atol=0)
+    assert abs(g.sum() / (2 * nx) ** 3 - og.sum() / (2 * nx) ** 3) < 1e-12
+    orho = orc.cicm1(opos, og, xxmin=xxmin)
+    np.testing.assert_array_equal(P.cicm1(opos, og, xxmin=xxmin), orho)
+    rh = P.cicm1(opos, g, xxmin=xxmin)
+    np.testing.assert_allclose(rh, orho, rtol=1e-12)
+    pdens, oisort, onn = orc.tri_cic(orho)
+    isort, nn = P.tri_cic(orho)
+    assert nn == onn
+    same_order_up_to_ties(isort, oisort, 1.0 / orho)
+    # the device entry point on resident densities gives the same order
+    import torch
+    disort, dnn = ctx.density_order_f64(torch.from_numpy(orho).to("cuda:0"))
+    np.testing.assert_array_equal(host(disort), isort)
+    assert dnn == nn
     with pytest.raises(P.PsodError):
         P.cic(opos, 2 * nx, mass=np.arange(len(opos), dtype=np.float32))
 
