@@ -210,3 +210,119 @@ def test_delta_conversions(orc):
     assert abs(orc.bryan_norman_mean(0.27) - 359.3) < 0.1
     assert abs(orc.delta_crit_to_mean(200.0, 0.27) - 740.74) < 0.01
     assert abs(orc.bryan_norman_mean(1.0) - 18 * np.pi ** 2) < 1e-12
+
+
+def halo_radii(z):
+    """Sorted FP64 distances of the halo particles from the centre, from
+    the stored float32 positions (the fixture's own numbers)."""
+    nx = int(z["nx"])
+    xs = z["pos"].astype(np.float64) * nx
+    d = np.abs(xs[z["labels"] == 0] - xs[int(z["centre"])])
+    d = np.minimum(d, nx - d)
+    return np.sort(np.sqrt((d * d).sum(axis=1)))
+
+
+def test_loglog_crossing_closed_form(orc):
+    """rint/mint: O(k) is an exact power law A r^-2 across the crossing, so
+    the log-log rule (haloanalyzer_new/src/modfunctions.f90:556-558) gives
+    r* = (A/Delta)^(1/2) and M* = Delta (4 pi/3) r*^3 in closed form.  Only
+    the float32 rounding of the positions (~3e-7 in r) separates them."""
+    z = load("powerlaw_crossing.npz")
+    r = fof_run(orc, z)
+    assert r["centre"][0] == 0 and r["robust_centre"][0] == 1
+    k = int(z["count"])
+    assert r["count"][0, 0] == k
+    assert abs(r["rint"][0, 0] / z["rint"] - 1) < 2e-6
+    assert abs(r["mint"][0, 0] / z["mint"] - 1) < 6e-6
+    # the same tolerances reject the plausible mistakes, computed from the
+    # fixture's own radii around the crossing
+    rs = halo_radii(z)
+    r1, r2 = rs[k - 1], rs[k]
+    assert r["radius"][0, 0] == r1
+    V0, D = 4.0 / 3.0 * np.pi, float(z["delta"])
+    o1, o2 = k / (V0 * r1 ** 3), (k + 1) / (V0 * r2 ** 3)
+    linear = r1 + (D - o1) * (r2 - r1) / (o2 - o1)
+    assert abs(linear / z["rint"] - 1) > 1e-4
+    # the count of the outer particle off by one
+    o2b = k / (V0 * r2 ** 3)
+    off = np.exp(np.log(r1) + (np.log(D) - np.log(o1)) *
+                 (np.log(r2) - np.log(r1)) / (np.log(o2b) - np.log(o1)))
+    assert abs(off / z["rint"] - 1) > 1e-3
+    # the discrete mass in place of the interpolated one
+    assert abs(k / z["mint"] - 1) > 1e-2
+
+
+def test_fof_com_across_periodic_edge(orc):
+    """com_pos: symmetric pairs around a centre 1/16 cell from the x = 0
+    face, every coordinate exact in float32: the centre of mass of the
+    sphere is the centre, bit for bit.  Averaging without the minimum-image
+    unwrap would put x near the middle of the box."""
+    z = load("com_edge.npz")
+    r = fof_run(orc, z)
+    assert r["centre"][0] == 0 and r["robust_centre"][0] == 1
+    assert r["count"][0, 0] == z["count"]
+    np.testing.assert_array_equal(r["com_pos"][0], z["com_pos"])
+    nx = int(z["nx"])
+    xs = z["pos"].astype(np.float64) * nx
+    d = np.abs(xs - xs[0])
+    d = np.minimum(d, nx - d)
+    inside = np.argsort(np.sqrt((d * d).sum(1)), kind="stable")
+    inside = inside[:int(z["count"])]
+    assert xs[inside, 0].max() - xs[inside, 0].min() > nx / 2  # straddles
+    naive = xs[inside, 0].mean() / nx
+    assert abs(naive - z["com_pos"][0]) > 0.1
+
+
+def test_greedy_exclusion_of_overlapping_halos(orc):
+    """Greedy exclusion (main file:2148-2166): halo A is grown first and
+    removes its sphere; halo B, 0.45 cells away, grows from the alive
+    particles only, so one of A's particles that lies inside B's radius is
+    not counted: B's count is its design, 27 (28 if A's sphere were not
+    excluded)."""
+    z = load("greedy_pair.npz")
+    pos, nx = z["pos"], int(z["nx"])
+    grid = orc.cic(pos, 2 * nx, xxmin=-0.5)
+    rho = orc.cicm1(pos, grid, xxmin=-0.5)
+    _, isort, nnew = orc.tri_cic(rho)
+    ids = np.arange(len(pos), dtype=np.int64)
+    r = orc.sod_greedy(pos, ids, isort, nnew,
+                       orc.greedy_params(nx, mmin=10.0, seuil2=800.0))
+    assert r["status"] == orc.OK
+    h = r["halos"]
+    ka = int(np.where(h["centre"] == z["centre_a"])[0][0])
+    kb = int(np.where(h["centre"] == z["centre_b"])[0][0])
+    assert ka < kb  # A's seed is denser and is processed first
+    assert h["masse"][ka] == z["count_a"] and h["masse"][kb] == z["count_b"]
+    assert h["rayon"][ka] == z["radius_a"] and h["rayon"][kb] == z["radius_b"]
+    assert h["accepted"][ka] == 1 and h["accepted"][kb] == 1
+    # both spheres are excluded afterwards; B's sphere is exactly its own
+    # particles
+    assert r["alive"][z["sphere_a"]].sum() == 0
+    assert r["alive"][z["sphere_b"]].sum() == 0
+    np.testing.assert_array_equal(np.sort(np.where(r["structure"] ==
+                                                   ids[z["centre_b"]])[0]),
+                                  z["sphere_b"])
+    # without the exclusion, B's sphere would hold one more particle
+    xs = pos.astype(np.float64) * nx
+    rb = np.sqrt(((xs - xs[int(z["centre_b"])]) ** 2).sum(1))
+    extra = np.setdiff1d(np.where(rb <= float(z["radius_b"]))[0],
+                         z["sphere_b"])
+    assert len(extra) == 1 and extra[0] in z["sphere_a"]
+
+
+def test_growth_at_given_centres_equals_search(orc):
+    """or_sod_fof_at with the centres or_sod_fof found gives its outputs."""
+    nx = 32
+    pos, vel, lab, ids, tab = orc.generate(
+        orc.gen_params(nx ** 3, 30, nlo=40, nhi=3000, seed=12))
+    grid = orc.cic(pos, nx)
+    rho = orc.cicm1(pos, grid)
+    seeds, nmem = orc.fof_seeds(lab, rho, len(tab["size"]))
+    cs, perm = orc.cell_sort(pos, nx, 8)
+    p = orc.fof_params(nx, len(pos), delta2=800.0)
+    a = orc.sod_fof(pos, cs, perm, 8, seeds, nmem, p, vel=vel)
+    b = orc.sod_fof(pos, cs, perm, 8, seeds, nmem, p, vel=vel,
+                    centres=a["centre"])
+    for k in ("centre", "count", "radius", "rint", "mint", "count2",
+              "profile", "com_pos", "com_vel", "nboundary", "w", "accepted"):
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
