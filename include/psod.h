@@ -66,6 +66,25 @@ int64_t psod_scratch_bytes(psod_ctx *ctx);
  * No reference counterpart: diagnostics of the execution paths. */
 int psod_sod_stats(psod_ctx *ctx, int64_t *out, int32_t n);
 
+/* ---- tunables and the autotuner (no reference counterpart) ----------------
+ * Thresholds of the batched pass that the autotuner chooses per device and
+ * workload.  0 in a field means the built-in default (shown in brackets).
+ *   small_max        particles in the 5^3 cells around the seed up to which a
+ *                    halo is done by one wavefront [1024, at most 1024]
+ *   bound_min        particles in the 5^3 cells from which the centre search
+ *                    bounds its boxes of 16 candidates before summing [2048]
+ *   dense_tile       own particles from which a CIC tile is dispatched first
+ *                    [16384]
+ *   small_wg_per_cu  one-wavefront halo workgroups per CU [8, at most 32]   */
+typedef struct {
+  int32_t small_max, bound_min, dense_tile, small_wg_per_cu;
+} psod_tunables;
+
+/* Sets / reads the context's tunables (those of psod_set_tunables override
+ * any loaded from a tuning file). */
+int psod_set_tunables(psod_ctx *ctx, const psod_tunables *t);
+int psod_get_tunables(psod_ctx *ctx, int32_t nx, psod_tunables *t);
+
 /* ---- a5: cell bucketing (main file:1561-1604) ----------------------------
  * Replaces the quick_sort of cell keys + run-length encoding with an integer
  * counting sort.  Cell of a particle: floor(x*nx) per axis, wrapped.  Cells
@@ -136,7 +155,7 @@ typedef struct {
   double rsearch;        /* centre-search radius in cells (1, :1525)          */
   double eps;            /* centre-search softening in cells (1/64, :1526)    */
   double tol_abs;        /* boundary listing: |r - R| <= tol_abs (cells)      */
-  int32_t small_max;     /* tunable: particles per one-wave halo (0: default) */
+  int32_t small_max;     /* overrides the small_max tunable when > 0         */
   int32_t flags;         /* PSOD_F_* below                                    */
 } psod_sod_params;
 
@@ -213,6 +232,51 @@ int psod_pass(psod_ctx *ctx, const psod_sod_params *p, int64_t npart,
               int32_t nx, int32_t ng, const float *d_pos, const float *d_vel,
               const int64_t *d_ids, const int32_t *d_labels, int64_t nhalo,
               float *d_grid, const psod_halo_out *out);
+
+/* ---- autotuner (no reference counterpart) -------------------------------
+ * Chooses the tunables (psod_tunables above) for one workload on the
+ * context's device: the FoF-seeded pass is prepared once on the caller's
+ * catalogue (as psod_pass does), then each tunable in turn is swept over its candidates with the
+ * others at their best so far, timing the stage it affects (CIC deposit or
+ * batched SOD, `reps` runs each, the fastest counts).  The choice becomes
+ * the context's tunables for grids of this nx; with cache_file not NULL it
+ * is also appended to that file as one line
+ *   psod-tune <device name> <CUs> <nx> <npart> <nhalo> <4 tunables> <ms>
+ * which psod_load_tunables reads back.  *chosen (may be NULL) receives it;
+ * stage_ms (may be NULL) receives [best deposit ms, best SOD ms]. */
+int psod_autotune(psod_ctx *ctx, const psod_sod_params *p, int64_t npart,
+                  int32_t nx, const float *d_pos, const float *d_vel,
+                  const int64_t *d_ids, const int32_t *d_labels,
+                  int64_t nhalo, int32_t reps, const char *cache_file,
+                  psod_tunables *chosen, double *stage_ms);
+
+/* Loads the lines of a tuning file written on a device with the same name
+ * and CU count: the last line for each nx applies to later calls on grids of
+ * that nx.  psod_create calls it with $PSOD_TUNE_FILE when that is set.
+ * Returns PSOD_OK with *nloaded = 0 when the file has no matching line. */
+int psod_load_tunables(psod_ctx *ctx, const char *cache_file,
+                       int32_t *nloaded);
+
+/* The same pass as psod_pass (same arguments, outputs and status codes),
+ * replayed as one HIP graph.  The reference's zone loop (main file:659-744)
+ * calls its routines one after the other; here the whole chain -- cell sort,
+ * CIC deposit, CIC^-1 and seeds, member counts, batched SOD with its second
+ * stream -- is captured once and then launched as one graph.
+ * The first call with a given set of arguments (pointers, sizes, params and
+ * the tunables in force) runs the pass eagerly, sizes every buffer, and
+ * captures the graph; later calls with the same arguments replay it (the
+ * inputs' contents may change between calls).  A replay reads back only the
+ * growth's failures; halos that need larger slices are then re-run
+ * eagerly, and a replay whose centre search needs more tasks than the
+ * capture allowed (twice those of the eager pass) is run again eagerly and
+ * captured anew on the next call.  *replayed (may be NULL) is 1 when this
+ * call replayed the graph.  Blocking, like psod_pass.  No reference
+ * counterpart for the graph itself. */
+int psod_pass_graph(psod_ctx *ctx, const psod_sod_params *p, int64_t npart,
+                    int32_t nx, int32_t ng, const float *d_pos,
+                    const float *d_vel, const int64_t *d_ids,
+                    const int32_t *d_labels, int64_t nhalo, float *d_grid,
+                    const psod_halo_out *out, int32_t *replayed);
 
 /* ---- synthetic catalogue (no reference counterpart: benchmark input) ------
  * Seeded NFW clumps on a uniform background; see or_gen_* in the oracle for

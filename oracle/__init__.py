@@ -91,7 +91,8 @@ class FofHalo(C.Structure):
         ("s_second", C.c_double), ("count", C.c_int64 * MAXDELTA),
         ("radius", C.c_double * MAXDELTA), ("overdens", C.c_double * MAXDELTA),
         ("rint", C.c_double * MAXDELTA), ("mint", C.c_double * MAXDELTA),
-        ("margin", C.c_double * MAXDELTA), ("count2", C.c_int64),
+        ("margin", C.c_double * MAXDELTA), ("rgap", C.c_double * MAXDELTA),
+        ("count2", C.c_int64),
         ("overdens2", C.c_double), ("margin2", C.c_double),
         ("profile_margin", C.c_double), ("nboundary", C.c_int64),
         ("com_pos", C.c_double * 3), ("com_vel", C.c_double * 3),

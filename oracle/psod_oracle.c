@@ -716,6 +716,9 @@ static int sod_fof_impl(int64_t npart, const float *pos, const float *vel,
           if (rel < mg) mg = rel;
         }
         o->margin[di] = mg;
+        double gp = (r2 - r1) / r1;
+        if (ks >= 3) gp = fmin(gp, (r1 - rr[ks - 3]) / r1);
+        o->rgap[di] = r1 > 0.0 ? gp : INFINITY;
       }
       /* second threshold, last count above delta2 (main file:2062-2069)    */
       o->count2 = 0;

@@ -127,6 +127,10 @@ typedef struct {
   double rint[OR_MAXDELTA];      /* log-log interpolated crossing (cells)     */
   double mint[OR_MAXDELTA];      /* Delta * 4pi/3 * rint^3 (particle masses)  */
   double margin[OR_MAXDELTA];    /* min_k |O(k)/Delta - 1| over the walk      */
+  double rgap[OR_MAXDELTA];      /* relative gap between R = r_{k*-1} and the */
+                                 /* sorted distances beside it (r_{k*-2},     */
+                                 /* r_{k*}): below float resolution another   */
+                                 /* particle may rank as the boundary one     */
   int64_t count2;
   double overdens2, margin2;
   double profile_margin;

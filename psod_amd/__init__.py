@@ -265,6 +265,57 @@ class Context:
                                 nhalo, _ptr(grid), C.byref(ho)))
         return o
 
+    # ---- tunables ----------------------------------------------------------
+    def set_tunables(self, small_max=0, bound_min=0, dense_tile=0,
+                     small_wg_per_cu=0):
+        """psod_set_tunables: 0 leaves a field to the tuning file or the
+        built-in default."""
+        t = _lib.Tunables(small_max, bound_min, dense_tile, small_wg_per_cu)
+        _lib.check(_L.psod_set_tunables(self.h, C.byref(t)))
+
+    def tunables(self, nx):
+        """The tunables in force for grids of nx cells per side."""
+        t = _lib.Tunables()
+        _lib.check(_L.psod_get_tunables(self.h, nx, C.byref(t)))
+        return {f: getattr(t, f) for f, _ in _lib.Tunables._fields_}
+
+    def autotune(self, p, pos, labels, nhalo, vel=None, ids=None, reps=2,
+                 cache_file=None):
+        """psod_autotune on a device catalogue: returns (chosen tunables,
+        best deposit ms, best SOD ms)."""
+        t = _lib.Tunables()
+        ms = (C.c_double * 2)()
+        self._after_torch()
+        _lib.check(_L.psod_autotune(
+            self.h, C.byref(p), pos.shape[0], p.nx, _ptr(pos), _ptr(vel),
+            _ptr(ids), _ptr(labels), nhalo, reps,
+            cache_file.encode() if cache_file else None, C.byref(t), ms))
+        return ({f: getattr(t, f) for f, _ in _lib.Tunables._fields_},
+                ms[0], ms[1])
+
+    def load_tunables(self, cache_file):
+        n = C.c_int32(0)
+        _lib.check(_L.psod_load_tunables(self.h, cache_file.encode(),
+                                         C.byref(n)))
+        return n.value
+
+    def run_pass_graph(self, p, pos, labels, nhalo, ng=None, vel=None,
+                       ids=None, grid=None, out=None):
+        """psod_pass_graph: the pass of run_pass, captured as one HIP graph
+        on the first call with these tensors and replayed afterwards.
+        Returns (outputs, replayed)."""
+        nx = p.nx
+        ng = nx if ng is None else ng
+        o = out if out is not None else self.alloc_halo_out(nhalo, p.nbins)
+        ho = _lib.HaloOut(**{k: _ptr(o[k]).value for k in o})
+        rp = C.c_int32(0)
+        self._after_torch()
+        _lib.check(_L.psod_pass_graph(self.h, C.byref(p), pos.shape[0], nx,
+                                      ng, _ptr(pos), _ptr(vel), _ptr(ids),
+                                      _ptr(labels), nhalo, _ptr(grid),
+                                      C.byref(ho), C.byref(rp)))
+        return o, bool(rp.value)
+
     def sod_stats(self):
         """Execution-path counts of the last sod_fof call: halos grown by
         one wave, by one workgroup, re-run with larger collection slices,

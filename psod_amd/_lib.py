@@ -23,6 +23,8 @@ EXPORTS = [
     "psod_sod_fof", "psod_sod_fof_plain", "psod_label_counts",
     "psod_gen_clumps", "psod_generate", "psod_host_cic", "psod_host_cicm1",
     "psod_host_tri_cic", "psod_host_spherover", "psod_density_order_f64",
+    "psod_set_tunables", "psod_get_tunables",
+    "psod_autotune", "psod_load_tunables", "psod_pass_graph",
 ]
 
 
@@ -71,6 +73,11 @@ class GenParams(C.Structure):
     ]
 
 
+class Tunables(C.Structure):
+    _fields_ = [("small_max", C.c_int32), ("bound_min", C.c_int32),
+                ("dense_tile", C.c_int32), ("small_wg_per_cu", C.c_int32)]
+
+
 class GreedyParams(C.Structure):
     _fields_ = [
         ("nx", C.c_int32), ("ncut", C.c_int32), ("seuil", C.c_double),
@@ -117,6 +124,8 @@ def load():
         "psod_label_counts": (C.c_int, [v, i64, v, i64, v]),
         "psod_pass": (C.c_int, [v, P(SodParams), i64, i32, i32, v, v, v, v,
                                 i64, v, P(HaloOut)]),
+        "psod_pass_graph": (C.c_int, [v, P(SodParams), i64, i32, i32, v, v,
+                                      v, v, i64, v, P(HaloOut), P(i32)]),
         "psod_gen_clumps": (i64, [P(GenParams), v, v, v, v, v, v]),
         "psod_generate": (C.c_int, [v, P(GenParams), v, v, v, v, v, v, v, v,
                                     v]),
@@ -124,6 +133,12 @@ def load():
         "psod_host_cicm1": (C.c_int, [v, i64, v, d, i32, v, v]),
         "psod_host_tri_cic": (C.c_int, [v, i64, v, d, v, P(i64)]),
         "psod_density_order_f64": (C.c_int, [v, i64, v, d, v, P(i64)]),
+        "psod_set_tunables": (C.c_int, [v, P(Tunables)]),
+        "psod_get_tunables": (C.c_int, [v, i32, P(Tunables)]),
+        "psod_autotune": (C.c_int, [v, P(SodParams), i64, i32, v, v, v, v,
+                                    i64, i32, C.c_char_p, P(Tunables),
+                                    P(d)]),
+        "psod_load_tunables": (C.c_int, [v, C.c_char_p, P(i32)]),
         "psod_host_spherover": (C.c_int, [v, i64, v, v, v, i64,
                                           P(GreedyParams), i64, P(i64), v, v,
                                           v, v, v, v]),

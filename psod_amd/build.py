@@ -15,6 +15,7 @@ LIB = os.path.join(HERE, "libpsod.so")
 FORTRAN = os.path.join(HERE, "fortran")
 
 SOURCES = ["ctx.hip", "gen.hip", "cellsort.hip", "cic.hip", "host_api.hip",
+           "tune.hip",
            "sod.hip", "sod_plain.hip", "greedy.hip", "pass.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

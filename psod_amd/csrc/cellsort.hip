@@ -204,6 +204,13 @@ k_cs_scatter(const float *__restrict__ pos, const int32_t *__restrict__ lab,
   }
 }
 
+__global__ void k_set_pair(int32_t *__restrict__ p, int32_t a, int32_t b) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    p[0] = a;
+    p[1] = b;
+  }
+}
+
 // tiles per bucket (entry nbk is not used by the scan's total)
 __global__ void k_cs_ntiles(const int32_t *__restrict__ bs, int32_t nbk,
                             int32_t *__restrict__ ntb) {
@@ -401,8 +408,10 @@ k_subsort(const int32_t *__restrict__ cs, const int32_t *__restrict__ list,
     if (it >= nl) break;
     const int32_t c = list[it];
     const int32_t a = cs[c], n = cs[c + 1] - a;
-    if (n > tcap || (n > SUB_REG && !trec)) {
-      // would not fit the slice: keep the scatter order
+    // cells up to SUB_REG records are done in registers (no slice: tcap
+    // is 0 then); larger ones need the staging slice
+    if (trec ? n > tcap : n > SUB_REG) {
+      // would not fit: keep the scatter order
       __syncthreads();
       continue;
     }
@@ -501,11 +510,8 @@ extern "C" int psod_cell_sort(psod_ctx *ctx, int64_t npart, const float *d_pos,
   int32_t *bs = bsA, *bs2 = bsB;
   int32_t nbk = 1;
   int cur = 0;  // buffer holding the last pass's output
-  {
-    const int32_t h0[2] = {0, (int32_t)n};
-    PSOD_HIP(hipMemcpyAsync(bs, h0, 8, hipMemcpyHostToDevice, st));
-    PSOD_HIP(hipStreamSynchronize(st));  // h0 lives on this stack frame
-  }
+  k_set_pair<<<1, 64, 0, st>>>(bs, 0, (int32_t)n);  // one bucket: [0, n)
+  PSOD_LAUNCH_CHECK();
   for (int p = 0; p < npass && n > 0; p++) {
     const int hi = bits - CS_DB * p;
     const int dbits = hi < CS_DB ? hi : CS_DB;
@@ -588,6 +594,5 @@ extern "C" int psod_cell_sort(psod_ctx *ctx, int64_t npart, const float *d_pos,
         (int32_t *)(tmp + (size_t)nsub * tcap * 20), tcap);
     PSOD_LAUNCH_CHECK();
   }
-  PSOD_HIP(hipStreamSynchronize(st));
-  return PSOD_OK;
+  return psod_finish(ctx);
 }

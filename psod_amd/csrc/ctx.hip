@@ -48,11 +48,26 @@ extern "C" int psod_create(int device, psod_ctx **out) {
     return psod_set_error(PSOD_E_DEVICE, "hipEventCreate failed");
   }
   hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, device) == hipSuccess)
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
     c->num_cus = prop.multiProcessorCount;
-  else
+    snprintf(c->device_name, sizeof(c->device_name), "%s", prop.gcnArchName);
+  } else {
     c->num_cus = 256;
+    snprintf(c->device_name, sizeof(c->device_name), "unknown");
+  }
+  // names are single words in the tuning file
+  for (char *q = c->device_name; *q; q++)
+    if (*q == ' ' || *q == '\t') *q = '_';
   *out = c;
+  if (const char *tf = getenv("PSOD_TUNE_FILE")) {
+    int32_t nl = 0;
+    const int rl = psod_load_tunables(c, tf, &nl);
+    if (rl != PSOD_OK) {
+      psod_destroy(c);
+      *out = nullptr;
+      return rl;
+    }
+  }
   return PSOD_OK;
 }
 
@@ -61,6 +76,7 @@ extern "C" int psod_destroy(psod_ctx *c) {
   hipSetDevice(c->device);
   hipStreamSynchronize(c->stream);
   hipStreamSynchronize(c->side);
+  psod_graph_release(c);
   for (int s = 0; s < S_NSLOT; s++)
     if (c->slot[s]) hipFree(c->slot[s]);
   hipEventDestroy(c->ev[0]);
@@ -97,15 +113,32 @@ extern "C" int psod_sod_stats(psod_ctx *c, int64_t *out, int32_t n) {
 
 int psod_use_device(psod_ctx *c) {
   if (!c) return psod_set_error(PSOD_E_ARG, "null context");
+  if (c->capturing) return PSOD_OK;  // the capturing call made it current
   PSOD_HIP(hipSetDevice(c->device));
+  return PSOD_OK;
+}
+
+int psod_finish(psod_ctx *c) {
+  if (c->capturing) return PSOD_OK;
+  PSOD_HIP(hipStreamSynchronize(c->stream));
   return PSOD_OK;
 }
 
 void *psod_scratch(psod_ctx *c, int slot, size_t bytes) {
   if (bytes == 0) bytes = 256;
   if (c->cap[slot] >= bytes) return c->slot[slot];
+  if (c->capturing) {
+    psod_set_error(PSOD_E_CAPACITY, "scratch slot %d would grow from %zu to "
+                   "%zu bytes while the pass is captured", slot, c->cap[slot],
+                   bytes);
+    return nullptr;
+  }
+  // a captured pass holds the old buffer's address: it is dropped (and
+  // captured again by the next psod_pass_graph call)
+  psod_graph_release(c);
   if (c->slot[slot]) {
     hipStreamSynchronize(c->stream);
+    hipStreamSynchronize(c->side);
     hipFree(c->slot[slot]);
     c->slot[slot] = nullptr;
     c->cap[slot] = 0;
@@ -249,9 +282,11 @@ int psod_scan_i32(psod_ctx *c, const int32_t *in, int32_t *out, int64_t n,
 constexpr int RS_T = 256, RS_V = 16, RS_TILE = RS_T * RS_V, RS_R = 256;
 
 __global__ void __launch_bounds__(RS_T)
-k_rs_hist(const uint64_t *__restrict__ keys, int64_t n, int shift,
+k_rs_hist(const uint64_t *__restrict__ keys, int64_t n,
+          const int32_t *__restrict__ dn, int shift,
           int32_t *__restrict__ hist, int64_t ntile) {
   __shared__ int32_t h[RS_R];
+  if (dn) n = min(n, (int64_t)*dn);
   h[threadIdx.x] = 0;
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * RS_TILE;
@@ -265,9 +300,10 @@ k_rs_hist(const uint64_t *__restrict__ keys, int64_t n, int shift,
 
 __global__ void __launch_bounds__(RS_T)
 k_rs_scatter(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
-             int64_t n, int shift, const int32_t *__restrict__ offs,
-             int64_t ntile, uint64_t *__restrict__ kout,
-             uint32_t *__restrict__ vout) {
+             int64_t n, const int32_t *__restrict__ dn, int shift,
+             const int32_t *__restrict__ offs, int64_t ntile,
+             uint64_t *__restrict__ kout, uint32_t *__restrict__ vout) {
+  if (dn) n = min(n, (int64_t)*dn);
   // Process the tile in RS_V rounds of 256 consecutive keys (one per
   // thread, thread order == index order), keeping a running count per digit.
   __shared__ int32_t run[RS_R];
@@ -315,7 +351,7 @@ k_rs_scatter(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
 }
 
 int psod_radix_sort_u64(psod_ctx *c, int64_t n, uint64_t *keys,
-                        uint32_t *vals, int key_bits) {
+                        uint32_t *vals, int key_bits, const int32_t *d_n) {
   if (n <= 1) return PSOD_OK;
   const int64_t ntile = (n + RS_TILE - 1) / RS_TILE;
   const int64_t hn = ntile * RS_R;
@@ -329,12 +365,12 @@ int psod_radix_sort_u64(psod_ctx *c, int64_t n, uint64_t *keys,
   int passes = (key_bits + 7) / 8;
   for (int p = 0; p < passes; p++) {
     const int shift = 8 * p;
-    k_rs_hist<<<(unsigned)ntile, RS_T, 0, c->stream>>>(ka, n, shift, hist,
-                                                       ntile);
+    k_rs_hist<<<(unsigned)ntile, RS_T, 0, c->stream>>>(ka, n, d_n, shift,
+                                                       hist, ntile);
     PSOD_LAUNCH_CHECK();
     PSOD_TRY(psod_scan_i32(c, hist, offs, hn, nullptr));
-    k_rs_scatter<<<(unsigned)ntile, RS_T, 0, c->stream>>>(ka, va, n, shift,
-                                                          offs, ntile, kb, vb);
+    k_rs_scatter<<<(unsigned)ntile, RS_T, 0, c->stream>>>(
+        ka, va, n, d_n, shift, offs, ntile, kb, vb);
     PSOD_LAUNCH_CHECK();
     uint64_t *tk = ka; ka = kb; kb = tk;
     uint32_t *tv = va; va = vb; vb = tv;

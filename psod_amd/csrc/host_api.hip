@@ -15,6 +15,13 @@
 
 #include "psod_internal.h"
 
+// hipcc contracts a*b + c into one fused operation by default.  Every FP64
+// expression of this file is evaluated as written, each operation rounded:
+// contraction is off here, and the operations are plain operators (HIP's
+// __dmul_rn and __dadd_rn are plain operators defined in a header, outside
+// this pragma, so they could still be fused once inlined).
+#pragma clang fp contract(off)
+
 int psod_fill_u64(psod_ctx *c, uint64_t *p, int64_t n, uint64_t v);
 
 // cell index, its periodic neighbour and the offset of one coordinate
@@ -25,10 +32,10 @@ int psod_fill_u64(psod_ctx *c, uint64_t *p, int64_t n, uint64_t v);
 __device__ inline void cic_index64(float x, double xxmin, double dx,
                                    int32_t ng, int32_t *i0, int32_t *i1,
                                    double *f) {
-  const double u = __ddiv_rn(__dsub_rn((double)x, xxmin), dx);
+  const double u = ((double)x - xxmin) / dx;
   int32_t ix = (int32_t)u;
   if (u < 0.0) ix = ix - 1;
-  *f = __dsub_rn(u, (double)ix);
+  *f = u - (double)ix;
   ix %= ng;
   if (ix < 0) ix += ng;
   *i0 = ix;
@@ -46,13 +53,13 @@ __device__ inline Corners64 corners64(const float *pos, int64_t p,
   cic_index64(pos[3 * p + 0], xxmin, dx, ng, &c.i[0], &c.i[1], &c.dd[0]);
   cic_index64(pos[3 * p + 1], xxmin, dx, ng, &c.j[0], &c.j[1], &c.dd[1]);
   cic_index64(pos[3 * p + 2], xxmin, dx, ng, &c.k[0], &c.k[1], &c.dd[2]);
-  for (int a = 0; a < 3; a++) c.de[a] = __dsub_rn(1.0, c.dd[a]);
+  for (int a = 0; a < 3; a++) c.de[a] = 1.0 - c.dd[a];
   return c;
 }
 
 // a * b * c * d evaluated left to right, each product rounded
 __device__ inline double prod4(double a, double b, double c, double d) {
-  return __dmul_rn(__dmul_rn(__dmul_rn(a, b), c), d);
+  return a * b * c * d;
 }
 
 // one particle per thread, eight FP64 atomic adds in the reference's corner
@@ -78,7 +85,7 @@ __global__ void k_cic64_scale(double *__restrict__ grid, int64_t n,
                               double vol) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
-    grid[i] = __ddiv_rn(grid[i], vol);
+    grid[i] = grid[i] / vol;
 }
 
 // rho = sum of the eight nodes times their weights, in the reference's order
@@ -95,8 +102,8 @@ __global__ void k_cic64_interp(const float *__restrict__ pos, int64_t n,
     for (int q = 0; q < 8; q++) {
       const int a = q & 1, b = (q >> 1) & 1, z = q >> 2;
       const double g = grid[((int64_t)c.k[z] * ng + c.j[b]) * ng + c.i[a]];
-      r = __dadd_rn(r, prod4(g, a ? c.dd[0] : c.de[0], b ? c.dd[1] : c.de[1],
-                             z ? c.dd[2] : c.de[2]));
+      r = r + prod4(g, a ? c.dd[0] : c.de[0], b ? c.dd[1] : c.de[1],
+                    z ? c.dd[2] : c.de[2]);
     }
     rho[p] = r;
   }
@@ -183,7 +190,7 @@ __global__ void k_pdens_keys(const double *__restrict__ rho, int64_t n,
   unsigned long long cnt = 0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    const double pd = __ddiv_rn(1.0, rho[i]);
+    const double pd = 1.0 / rho[i];
     // rho > 0 for every particle (its own mass is deposited); a
     // non-positive or NaN value sorts last
     keys[i] = pd > 0.0 ? (uint64_t)__double_as_longlong(pd) : ~0ull;

@@ -67,11 +67,27 @@ enum ScratchSlot {
   S_PASS_LAB,
   S_PASS_HALO,
   S_PASS_GRID,
+  S_TUNE,      // autotuner: per-halo outputs
   S_NSLOT
 };
 
+// tunables loaded from a tuning file, one entry per grid size
+constexpr int PSOD_TUNE_ENTRIES = 16;
+struct TuneEntry {
+  int32_t nx;
+  psod_tunables t;
+};
+
+// built-in defaults of the tunables
+constexpr int32_t TUNE_SMALL_MAX = 1024, TUNE_BOUND_MIN = 2048,
+                  TUNE_DENSE_TILE = 1 << 14, TUNE_SMALL_WG = 8;
+
 struct psod_ctx {
   int device;
+  psod_tunables tune_set;   // psod_set_tunables; 0 fields are unset
+  TuneEntry tune_file[PSOD_TUNE_ENTRIES];
+  int ntune_file;
+  char device_name[256];
   hipStream_t stream;
   void *slot[S_NSLOT];
   size_t cap[S_NSLOT];
@@ -82,7 +98,37 @@ struct psod_ctx {
   // and the events that fork to it and join from it
   hipStream_t side;
   hipEvent_t fork, join;
+  // set while psod_pass_graph captures the pass: entry points queue their
+  // work without waiting, device scratch may not grow, and the SOD reads
+  // its work counts on the device (GraphCaps bounds them)
+  int capturing;
+  struct GraphCaps {
+    int64_t tasks;  // centre-search tasks of one pass, at most
+  } gcap;
+  struct GraphState *graph;  // psod_pass_graph's captured pass (or NULL)
 };
+
+// Frees psod_pass_graph's captured pass of the context (psod_destroy).
+void psod_graph_release(psod_ctx *ctx);
+
+// End of an entry point: waits for the context's stream, except while the
+// pass is being captured.
+int psod_finish(psod_ctx *ctx);
+
+// intermediates of the FoF-seeded pass (context scratch)
+struct PassBuffers {
+  int32_t *cell_start;  // [nx^3 + 1]
+  float *rec;           // [4 npart]
+  int32_t *lab_sorted;  // [npart]
+  uint64_t *seed_key;   // [nhalo]
+  int32_t *seed, *nmem; // [nhalo]
+  float *grid;          // [ng^3]
+};
+
+// Tunables in force for grids of nx cells per side: the built-in defaults,
+// overridden by a loaded tuning-file entry for nx, overridden by the fields
+// set with psod_set_tunables; every field is > 0 and within its range.
+psod_tunables psod_tunables_for(const psod_ctx *ctx, int32_t nx);
 
 // Returns a device buffer of at least `bytes` in `slot` (contents not kept
 // across growth) or nullptr with the error set.
@@ -131,9 +177,12 @@ inline unsigned grid_for(int64_t n, int block, int64_t maxblocks = 1 << 30) {
 int psod_scan_i32(psod_ctx *ctx, const int32_t *in, int32_t *out, int64_t n,
                   int32_t *total);
 
-// radix sort of 64-bit keys with 32-bit values, ascending, stable.
+// radix sort of 64-bit keys with 32-bit values, ascending, stable.  With
+// d_n not NULL only the first min(n, *d_n) elements are sorted (a count
+// known on the device only; n is then the capacity).
 int psod_radix_sort_u64(psod_ctx *ctx, int64_t n, uint64_t *keys,
-                        uint32_t *vals, int key_bits);
+                        uint32_t *vals, int key_bits,
+                        const int32_t *d_n = nullptr);
 
 // ---------------------------------------------------------------------------
 // wave64 helpers

@@ -149,7 +149,7 @@ k_plan(SodCfg c, int64_t nhalo, const int32_t *__restrict__ seed,
   for (int d = 0; d < 3; d++) pl.cs[d] = sr.c[d];
   pl.n5 = n5;
   pl.w0 = reach_estimate(c, pl.nmem);
-  pl.klass = (n5 <= CAP5 && pl.w0 <= WMAX_S) ? 0 : 1;
+  pl.klass = (n5 <= c.small_max && pl.w0 <= WMAX_S) ? 0 : 1;
   plan[h] = pl;
 }
 
@@ -592,9 +592,16 @@ __device__ void small_growth_and_output(const SodCfg &c, SmallLds &S,
 __global__ void __launch_bounds__(256)
 k_task_count(SodCfg c, const int32_t *__restrict__ cell_start,
              HaloPlan *__restrict__ plan, const int32_t *__restrict__ large,
-             int32_t nlarge, int32_t *__restrict__ ntask) {
+             int32_t nlarge, const int32_t *__restrict__ dn,
+             int32_t *__restrict__ ntask) {
+  // nlarge bounds the launch; with dn the halos are the first *dn (entries
+  // up to nlarge beyond them count no tasks, for the scan)
   const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nlarge) return;
+  if (dn && e >= *dn) {
+    ntask[e] = 0;
+    return;
+  }
   const HaloPlan pl = plan[large[e]];
   int32_t nt = 0;
   if (pl.centre < 0) {
@@ -613,14 +620,18 @@ k_task_count(SodCfg c, const int32_t *__restrict__ cell_start,
 __global__ void __launch_bounds__(256)
 k_task_emit(SodCfg c, const int32_t *__restrict__ cell_start,
             HaloPlan *__restrict__ plan, const int32_t *__restrict__ large,
-            int32_t nlarge, const int32_t *__restrict__ ntask,
-            const int32_t *__restrict__ task0, int4 *__restrict__ tasks) {
+            int32_t nlarge, const int32_t *__restrict__ dn,
+            const int32_t *__restrict__ ntask,
+            const int32_t *__restrict__ task0,
+            const int32_t *__restrict__ overflow, int4 *__restrict__ tasks) {
   const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nlarge) return;
+  if (e >= nlarge || (dn && e >= *dn)) return;
   const int32_t h = large[e];
-  plan[h].ntask = ntask[e];
-  plan[h].task0 = task0[e];
-  if (ntask[e] == 0) return;
+  // tasks beyond the capacity of a captured pass: none are written
+  const bool none = overflow && *overflow;
+  plan[h].ntask = none ? 0 : ntask[e];
+  plan[h].task0 = none ? 0 : task0[e];
+  if (none || ntask[e] == 0) return;
   const HaloPlan pl = plan[h];
   int32_t t = task0[e];
   for (int q = 0; q < 27; q++) {
@@ -670,9 +681,9 @@ __constant__ int8_t kCellOrder[27] = {13, 4,  10, 12, 14, 16, 22, 1,  3,
                                       5,  7,  9,  11, 15, 17, 19, 21, 23,
                                       25, 0,  2,  6,  8,  18, 20, 24, 26};
 
-// halos with fewer particles in the 5^3 cells around the seed skip the
-// bound: bounding costs about as much as their exact sums
-constexpr int32_t BB_MIN = 2048;
+// Halos with fewer than SodCfg::bound_min particles in the 5^3 cells around
+// the seed (a tunable, 2048 by default) skip the bound: bounding costs about
+// as much as their exact sums.
 
 // Upper bound of the term of any pair at squared distance >= g2 within the
 // cutoff.  The float d2 of a pair is at least g2 up to a few ulps, hence the
@@ -728,7 +739,7 @@ __global__ void __launch_bounds__(256)
 k_centre_bound(SodCfg c, const int32_t *__restrict__ cell_start,
                const float4 *__restrict__ rec, const HaloPlan *__restrict__ plan,
                const int32_t *__restrict__ large, const int4 *__restrict__ tasks,
-               const int32_t *__restrict__ ntask_total,
+               const int32_t *__restrict__ ntask_total, int32_t tcap,
                unsigned long long *__restrict__ hpil,
                unsigned long long *__restrict__ ubt,
                unsigned long long *__restrict__ cub, int32_t *__restrict__ wq) {
@@ -745,7 +756,7 @@ k_centre_bound(SodCfg c, const int32_t *__restrict__ cell_start,
     for (int32_t t = t0; t < t1; t++) {
       const int4 tk = tasks[t];
       const HaloPlan pl = plan[large[tk.x]];
-      if (pl.n5 < BB_MIN) {
+      if (pl.n5 < c.bound_min) {
         // few particles around the seed: every task of the halo is evaluated
         // exactly by the pilots (an all-ones bound makes each one a pilot)
         if (l == 0) {
@@ -895,7 +906,7 @@ k_centre_bound(SodCfg c, const int32_t *__restrict__ cell_start,
         q = qn;
         s = sn;
       }
-      if ((l & 7) == 0) cub[((int64_t)nt * 27 + t) * 8 + (l >> 3)] = tot;
+      if ((l & 7) == 0) cub[((int64_t)tcap * 27 + t) * 8 + (l >> 3)] = tot;
       unsigned long long ubmax = (l >> 3) < ngr ? tot : 0ull;
       for (int o = 32; o > 0; o >>= 1) {
         const unsigned long long v = __shfl_xor(ubmax, o, 64);
@@ -1072,7 +1083,7 @@ k_centre_pilot(SodCfg c, const int32_t *__restrict__ cell_start,
 // with it.
 __global__ void __launch_bounds__(256)
 k_task_items(const int4 *__restrict__ tasks,
-             const int32_t *__restrict__ ntask_total,
+             const int32_t *__restrict__ ntask_total, int32_t tcap,
              const unsigned long long *__restrict__ ubt,
              const unsigned long long *__restrict__ hpil,
              const unsigned long long *__restrict__ cub,
@@ -1091,7 +1102,7 @@ k_task_items(const int4 *__restrict__ tasks,
       const int ngr = (tk.w + 7) >> 3;
       if (ub >= best)
         for (int g = 0; g < ngr; g += 2)
-          if (cub[((int64_t)nt * 27 + t) * 8 + g] >= best) mask |= 1u << g;
+          if (cub[((int64_t)tcap * 27 + t) * 8 + g] >= best) mask |= 1u << g;
       skipped = mask == 0;
     }
   }
@@ -1116,15 +1127,16 @@ __global__ void k_item_keys(const int32_t *__restrict__ items,
                             const int32_t *__restrict__ nitems,
                             const int4 *__restrict__ tasks,
                             const int32_t *__restrict__ ntask_total,
+                            int32_t tcap,
                             const unsigned long long *__restrict__ cub,
                             const unsigned long long *__restrict__ hbest,
                             uint64_t *__restrict__ keys,
                             uint32_t *__restrict__ vals) {
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= *nitems) return;
-  const int32_t nt = *ntask_total, it = items[i];
+  const int32_t it = items[i];
   const int32_t t = it >> 3, g = it & 7;
-  const double ub = (double)cub[((int64_t)nt * 27 + t) * 8 + g];
+  const double ub = (double)cub[((int64_t)tcap * 27 + t) * 8 + g];
   const double b = (double)hbest[tasks[t].x];
   const double r = b > 0.0 ? ub / b : 1e9;
   const double q = fmin(r, 255.0) * 16777216.0;  // 2^24 per unit of ratio
@@ -1154,7 +1166,7 @@ __global__ void __launch_bounds__(256, 5)
 k_centre_group(SodCfg c, const int32_t *__restrict__ cell_start,
                const float4 *__restrict__ rec, const HaloPlan *__restrict__ plan,
                const int32_t *__restrict__ large, const int4 *__restrict__ tasks,
-               const int32_t *__restrict__ ntask_total,
+               const int32_t *__restrict__ ntask_total, int32_t tcap,
                const int32_t *__restrict__ items,
                const int32_t *__restrict__ nitems,
                unsigned long long *__restrict__ hbest,
@@ -1182,7 +1194,7 @@ k_centre_group(SodCfg c, const int32_t *__restrict__ cell_start,
       uint64_t *res = itemres + 2 * (int64_t)it;
       unsigned long long best = __hip_atomic_load(
           &hbest[tk.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (cub[((int64_t)nt * 27 + t) * 8 + g] < best) {
+      if (cub[((int64_t)tcap * 27 + t) * 8 + g] < best) {
         if (l == 0) {
           res[0] = 0;
           res[1] = ~0ull;
@@ -1410,11 +1422,12 @@ __global__ void k_res_init(uint64_t *__restrict__ r, int64_t n) {
 // One wave per halo, its lanes over the tasks.
 __global__ void __launch_bounds__(256)
 k_centre_reduce(HaloPlan *__restrict__ plan, const int32_t *__restrict__ large,
-                int32_t nlarge, const uint64_t *__restrict__ taskres,
+                int32_t nlarge, const int32_t *__restrict__ dn,
+                const uint64_t *__restrict__ taskres,
                 const uint64_t *__restrict__ itemres) {
   const int l = threadIdx.x & 63;
   const int32_t e = blockIdx.x * 4 + (int32_t)(threadIdx.x >> 6);
-  if (e >= nlarge) return;  // uniform per wave
+  if (e >= nlarge || (dn && e >= *dn)) return;  // uniform per wave
   const int32_t h = large[e];
   const HaloPlan pl = plan[h];
   if (pl.centre >= 0 || pl.ntask == 0) return;
@@ -1794,13 +1807,17 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
              uint64_t *__restrict__ pool_all,
              int64_t pool_cap, const int32_t *__restrict__ elist,
              int32_t *__restrict__ retry,
-             unsigned long long *__restrict__ needmax, psod_halo_out out) {
+             unsigned long long *__restrict__ needmax,
+             const int32_t *__restrict__ abort_flag, psod_halo_out out) {
   // Work items are positions e of the large list (or elist[item] when a
   // list of positions is given).  A halo whose bands do not fit the slice
   // is appended to `retry` (when given) with the slice size it needs
   // folded into *needmax, for a second launch with larger slices.
   __shared__ GrowShared G;
   __shared__ int32_t lh[HIST_LDS];  // inner bins; staging of the scan
+  // a captured pass whose centre search overflowed: the host runs the pass
+  // again without the graph (uniform per workgroup, before any barrier)
+  if (abort_flag && *abort_flag) return;
   const int32_t nitem = counters[1];
   // this workgroup's slices of the collection pool and of the radial
   // histogram, reused for every halo
@@ -2295,12 +2312,13 @@ __global__ void k_retry_reset(int32_t *__restrict__ c, int32_t nitem,
 // beside a giant one walks the giant's particles too.
 __global__ void k_large_keys(SodCfg c, const HaloPlan *__restrict__ plan,
                              const int32_t *__restrict__ large, int32_t n,
+                             const int32_t *__restrict__ dn,
                              const int32_t *__restrict__ cell_start,
                              int growth,
                              uint64_t *__restrict__ keys,
                              uint32_t *__restrict__ vals) {
   const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
+  if (e >= n || (dn && e >= *dn)) return;
   const int32_t h = large[e];
   const HaloPlan pl = plan[h];
   int64_t cost = (int64_t)max(pl.nmem, 0) + pl.n5;
@@ -2331,243 +2349,203 @@ __global__ void k_large_keys(SodCfg c, const HaloPlan *__restrict__ plan,
   vals[e] = (uint32_t)h;
 }
 
-// Centre search of the first nlarge halos of `large` (positions e): tasks
-// of 64 candidates, their bounds, pilots, groups, and each halo's centre in
-// plan[].centre.  ntask, task0 and ntot are scratch of nlarge + 16 entries
-// (ntot: 16); hbest holds 2 * nlarge + 4 entries.
-// after_setup() is called once the task count is known, before the
-// search's kernels are queued.
+// Clamps the task count of a captured pass to the buffers' capacity and
+// flags the overflow (the pass then computes nothing for the large halos
+// and the host runs it again without the graph).
+__global__ void k_task_cap(int32_t *__restrict__ ntot, int32_t tcap,
+                           int32_t *__restrict__ overflow) {
+  if (blockIdx.x == 0 && threadIdx.x == 0 && *ntot > tcap) {
+    *overflow = 1;
+    *ntot = 0;
+  }
+}
+
+__global__ void k_copy_i32(const int32_t *__restrict__ a,
+                           int32_t *__restrict__ b) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *b = *a;
+}
+
+// Work counts of a pass.  Eager calls read them back on the host and size
+// every launch exactly.  A pass that psod_pass_graph captures cannot read
+// anything back: its launches are sized by bounds (the halo count, and a
+// centre-search task capacity from an earlier eager pass) and its kernels
+// read the counts on the device.  counters[36] is set when that pass's
+// tasks exceed the capacity (the host then runs the pass without the
+// graph), counters[37] holds the large-halo count before the small path's
+// deferrals.
+struct SodCounts {
+  bool dev;         // counts on the device only (captured pass)
+  int32_t nlarge1;  // large halos of the plan (eager)
+  int32_t nlarge;   // large halos after the small path (eager)
+};
+
+// Centre search of the first nlarge halos of `large` (positions e; with
+// dn, the first *dn of at most nlarge): tasks of 64 candidates, their
+// bounds, pilots, groups, and each halo's centre in plan[].centre.  ntask,
+// task0 and ntot are scratch of nlarge + 16 entries (ntot: 16); hbest
+// holds 2 * nlarge + 4 entries.  after_setup() is called once the task
+// count is known, before the search's kernels are queued.
 template <typename F>
 static int centre_search(psod_ctx *ctx, const SodCfg &c,
                          const psod_catalogue *cat, HaloPlan *plan,
-                         const int32_t *large, int32_t nlarge, int32_t *ntask,
+                         const int32_t *large, int32_t nlarge,
+                         const int32_t *dn, int32_t *overflow, int32_t *ntask,
                          int32_t *task0, int32_t *ntot,
                          unsigned long long *hbest, F &&after_setup) {
   hipStream_t st = ctx->stream;
   k_task_count<<<grid_for(nlarge, 256), 256, 0, st>>>(c, cat->cell_start,
-                                                       plan, large, nlarge,
+                                                       plan, large, nlarge, dn,
                                                        ntask);
   PSOD_LAUNCH_CHECK();
   PSOD_TRY(psod_scan_i32(ctx, ntask, task0, nlarge, ntot));
-  int32_t nt = 0;
-  PSOD_HIP(hipMemcpyAsync(&nt, ntot, 4, hipMemcpyDeviceToHost, st));
-  PSOD_HIP(hipStreamSynchronize(st));
+  // tcap: tasks the buffers hold, and the stride of their layout
+  int32_t nt = 0, tcap = 0;
+  if (dn) {
+    tcap = (int32_t)std::min<int64_t>(ctx->gcap.tasks, 1 << 27);
+    k_task_cap<<<1, 64, 0, st>>>(ntot, tcap, overflow);
+    PSOD_LAUNCH_CHECK();
+  } else {
+    PSOD_HIP(hipMemcpyAsync(&nt, ntot, 4, hipMemcpyDeviceToHost, st));
+    PSOD_HIP(hipStreamSynchronize(st));
+    if (nt > (1 << 27))
+      return psod_set_error(PSOD_E_CAPACITY,
+                            "%d centre-search tasks exceed 2^27", nt);
+    tcap = nt;
+    ctx->sod_stat[7] = nt;
+  }
   PSOD_TRY(after_setup());
-  if (nt > (1 << 27))
-    return psod_set_error(PSOD_E_CAPACITY,
-                          "%d centre-search tasks exceed 2^27", nt);
-  if (nt > 0) {
-    // tasks, their results, bounds, bounds per (cell, group) plus the
-    // per-group totals (28 x 8 per task), the results of the groups (8 per
-    // task), the work list of groups and its length
-    int4 *tasks = (int4 *)psod_scratch(
-        ctx, S_TASKRES,
-        (size_t)nt * (sizeof(int4) + 24 + 28 * 8 * 8 + 8 * 16 + 8 * 4) + 256);
-    if (!tasks) return PSOD_E_NOMEM;
-    uint64_t *tres = (uint64_t *)(tasks + nt);
-    unsigned long long *ubt = (unsigned long long *)(tres + 2 * (int64_t)nt);
-    unsigned long long *cub = ubt + nt;
-    uint64_t *ires = (uint64_t *)(cub + (int64_t)nt * 28 * 8);
-    int32_t *items = (int32_t *)(ires + 16 * (int64_t)nt);
-    int32_t *nitems = items + 8 * (int64_t)nt;
-    int32_t *wq = nitems + 4;  // work counters of the bound, pilot, group
-    k_task_emit<<<grid_for(nlarge, 256), 256, 0, st>>>(
-        c, cat->cell_start, plan, large, nlarge, ntask, task0, tasks);
+  if (tcap == 0) return PSOD_OK;
+  // tasks, their results, bounds, bounds per (cell, group) plus the
+  // per-group totals (28 x 8 per task), the results of the groups (8 per
+  // task), the work list of groups and its length
+  int4 *tasks = (int4 *)psod_scratch(
+      ctx, S_TASKRES,
+      (size_t)tcap * (sizeof(int4) + 24 + 28 * 8 * 8 + 8 * 16 + 8 * 4) + 256);
+  if (!tasks) return PSOD_E_NOMEM;
+  uint64_t *tres = (uint64_t *)(tasks + tcap);
+  unsigned long long *ubt = (unsigned long long *)(tres + 2 * (int64_t)tcap);
+  unsigned long long *cub = ubt + tcap;
+  uint64_t *ires = (uint64_t *)(cub + (int64_t)tcap * 28 * 8);
+  int32_t *items = (int32_t *)(ires + 16 * (int64_t)tcap);
+  int32_t *nitems = items + 8 * (int64_t)tcap;
+  int32_t *wq = nitems + 4;  // work counters of the bound, pilot, group
+  k_task_emit<<<grid_for(nlarge, 256), 256, 0, st>>>(
+      c, cat->cell_start, plan, large, nlarge, dn, ntask, task0,
+      dn ? overflow : nullptr, tasks);
+  PSOD_LAUNCH_CHECK();
+  k_res_init<<<grid_for(tcap, 256), 256, 0, st>>>(tres, tcap);
+  PSOD_LAUNCH_CHECK();
+  k_res_init<<<grid_for(8 * (int64_t)tcap, 256), 256, 0, st>>>(
+      ires, 8 * (int64_t)tcap);
+  PSOD_LAUNCH_CHECK();
+  // hbest[nlarge..nlarge+3]: units evaluated, units skipped, pair tests,
+  // units abandoned; then each halo's largest task bound
+  unsigned long long *cstat = hbest + nlarge;
+  unsigned long long *hpil = cstat + 4;
+  PSOD_TRY(psod_fill_u64(ctx, (uint64_t *)hbest, 2 * (int64_t)nlarge + 4, 0));
+  PSOD_TRY(psod_fill_i32(ctx, nitems, 8, 0));
+  if (!ctx->capturing) PSOD_HIP(hipEventRecord(ctx->ev[0], st));
+  // as many workgroups as fit at once; their waves take work from counters
+  auto resident = [&](const void *k) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, 256, 0) !=
+            hipSuccess ||
+        nb < 1)
+      nb = 4;
+    return (unsigned)ctx->num_cus * (unsigned)nb;
+  };
+  const unsigned nwg = resident((const void *)k_centre_bound);
+  k_centre_bound<<<nwg, 256, 0, st>>>(c, cat->cell_start,
+                                      (const float4 *)cat->rec, plan, large,
+                                      tasks, ntot, tcap, hpil, ubt, cub, wq);
+  PSOD_LAUNCH_CHECK();
+  k_centre_pilot<<<resident((const void *)k_centre_pilot), 256, 0, st>>>(
+      c, cat->cell_start, (const float4 *)cat->rec, plan, large, tasks, ntot,
+      tres, hbest, hpil, ubt, cstat, wq + 1);
+  PSOD_LAUNCH_CHECK();
+  k_task_items<<<grid_for(tcap, 256), 256, 0, st>>>(
+      tasks, ntot, tcap, ubt, hpil, cub, hbest, items, nitems, cstat);
+  PSOD_LAUNCH_CHECK();
+  // items sorted by their bound over the halo's best, largest first
+  int32_t ni = 8 * tcap;  // capacity (device count) or the exact count
+  if (!dn) {
+    PSOD_HIP(hipMemcpyAsync(&ni, nitems, 4, hipMemcpyDeviceToHost, st));
+    PSOD_HIP(hipStreamSynchronize(st));
+  }
+  if (ni > 1) {
+    uint64_t *keys = (uint64_t *)psod_scratch(ctx, S_KEYS, (size_t)ni * 8);
+    uint32_t *vals = (uint32_t *)psod_scratch(ctx, S_VALS, (size_t)ni * 4);
+    if (!keys || !vals) return PSOD_E_NOMEM;
+    k_item_keys<<<grid_for(ni, 256), 256, 0, st>>>(items, nitems, tasks, ntot,
+                                                   tcap, cub, hbest, keys,
+                                                   vals);
     PSOD_LAUNCH_CHECK();
-    k_res_init<<<grid_for(nt, 256), 256, 0, st>>>(tres, nt);
-    PSOD_LAUNCH_CHECK();
-    k_res_init<<<grid_for(8 * (int64_t)nt, 256), 256, 0, st>>>(
-        ires, 8 * (int64_t)nt);
-    PSOD_LAUNCH_CHECK();
-    // hbest[nlarge..nlarge+3]: units evaluated, units skipped, pair tests,
-    // units abandoned; then each halo's largest task bound
-    unsigned long long *cstat = hbest + nlarge;
-    unsigned long long *hpil = cstat + 4;
-    PSOD_TRY(psod_fill_u64(ctx, (uint64_t *)hbest, 2 * (int64_t)nlarge + 4, 0));
-    PSOD_TRY(psod_fill_i32(ctx, nitems, 8, 0));
-    PSOD_HIP(hipEventRecord(ctx->ev[0], st));
-    // as many workgroups as fit at once; their waves take work from counters
-    auto resident = [&](const void *k) {
-      int nb = 0;
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, 256, 0) !=
-              hipSuccess ||
-          nb < 1)
-        nb = 4;
-      return (unsigned)ctx->num_cus * (unsigned)nb;
-    };
-    const unsigned nwg = resident((const void *)k_centre_bound);
-    k_centre_bound<<<nwg, 256, 0, st>>>(c, cat->cell_start,
-                                        (const float4 *)cat->rec, plan, large,
-                                        tasks, ntot, hpil, ubt, cub, wq);
-    PSOD_LAUNCH_CHECK();
-    k_centre_pilot<<<resident((const void *)k_centre_pilot), 256, 0, st>>>(c, cat->cell_start,
-                                        (const float4 *)cat->rec, plan, large,
-                                        tasks, ntot, tres, hbest, hpil, ubt,
-                                        cstat, wq + 1);
-    PSOD_LAUNCH_CHECK();
-    k_task_items<<<grid_for(nt, 256), 256, 0, st>>>(tasks, ntot, ubt, hpil, cub,
-                                                    hbest, items, nitems, cstat);
-    PSOD_LAUNCH_CHECK();
-    {
-      int32_t ni = 0;
-      PSOD_HIP(hipMemcpyAsync(&ni, nitems, 4, hipMemcpyDeviceToHost, st));
-      PSOD_HIP(hipStreamSynchronize(st));
-      if (ni > 1) {
-        uint64_t *keys = (uint64_t *)psod_scratch(ctx, S_KEYS, (size_t)ni * 8);
-        uint32_t *vals = (uint32_t *)psod_scratch(ctx, S_VALS, (size_t)ni * 4);
-        if (!keys || !vals) return PSOD_E_NOMEM;
-        k_item_keys<<<grid_for(ni, 256), 256, 0, st>>>(items, nitems, tasks, ntot,
-                                                       cub, hbest, keys, vals);
-        PSOD_LAUNCH_CHECK();
-        PSOD_TRY(psod_radix_sort_u64(ctx, ni, keys, vals, 32));
-        PSOD_HIP(hipMemcpyAsync(items, vals, (size_t)ni * 4,
-                                hipMemcpyDeviceToDevice, st));
-      }
-    }
-    k_centre_group<<<resident((const void *)k_centre_group), 256, 0, st>>>(
-        c, cat->cell_start, (const float4 *)cat->rec, plan, large, tasks, ntot,
-        items, nitems, hbest, cub, ires, cstat, wq + 2);
-    PSOD_LAUNCH_CHECK();
-    k_centre_reduce<<<grid_for(nlarge, 4), 256, 0, st>>>(plan, large, nlarge,
-                                                         tres, ires);
-    PSOD_LAUNCH_CHECK();
+    PSOD_TRY(psod_radix_sort_u64(ctx, ni, keys, vals, 32, dn ? nitems : nullptr));
+    PSOD_HIP(hipMemcpyAsync(items, vals, (size_t)ni * 4,
+                            hipMemcpyDeviceToDevice, st));
+  }
+  k_centre_group<<<resident((const void *)k_centre_group), 256, 0, st>>>(
+      c, cat->cell_start, (const float4 *)cat->rec, plan, large, tasks, ntot,
+      tcap, items, nitems, hbest, cub, ires, cstat, wq + 2);
+  PSOD_LAUNCH_CHECK();
+  k_centre_reduce<<<grid_for(nlarge, 4), 256, 0, st>>>(plan, large, nlarge, dn,
+                                                       tres, ires);
+  PSOD_LAUNCH_CHECK();
+  if (!ctx->capturing) {
     PSOD_HIP(hipEventRecord(ctx->ev[1], st));
     // read back with the growth's counters (the stream is synchronised
-    // before this call returns)
+    // before the pass returns)
     PSOD_HIP(hipMemcpyAsync(ctx->sod_stat + 4, cstat, 3 * 8,
                             hipMemcpyDeviceToHost, st));
     PSOD_HIP(hipMemcpyAsync(ctx->sod_stat + 9, cstat + 3, 8,
                             hipMemcpyDeviceToHost, st));
-    ctx->sod_stat[7] = nt;
   }
   return PSOD_OK;
 }
 
-static int sod_fof_run(psod_ctx *ctx, const psod_sod_params *p,
-                       const psod_catalogue *cat, int64_t nhalo,
-                       const int32_t *d_seed, const int32_t *d_nmem,
-                       const psod_halo_out *out) {
-  PSOD_TRY(check_sod_args(p, cat, nhalo, d_seed, out));
-  if (nhalo <= 0) return PSOD_OK;
-  if (!out->profile)
-    return psod_set_error(PSOD_E_ARG, "sod: profile output is required");
-  const SodCfg c = make_cfg(p, cat->npart);
-  hipStream_t st = ctx->stream;
-  // plan, lists and counters
-  HaloPlan *plan = (HaloPlan *)psod_scratch(ctx, S_PLAN,
-                                            (size_t)nhalo * sizeof(HaloPlan));
+// The scratch of one pass: plan, lists, counters and the centre search's
+// per-halo arrays.
+struct SodScratch {
+  HaloPlan *plan;
+  int32_t *small, *large, *counters;
+  int32_t *ntask, *task0, *ntot, *retry;
+  unsigned long long *hbest;
+};
+
+static int sod_scratch(psod_ctx *ctx, int64_t nhalo, SodScratch *S) {
+  S->plan = (HaloPlan *)psod_scratch(ctx, S_PLAN,
+                                     (size_t)nhalo * sizeof(HaloPlan));
   int32_t *lists = (int32_t *)psod_scratch(ctx, S_TASK,
                                            (size_t)(2 * nhalo + 64) * 4);
-  if (!plan || !lists) return PSOD_E_NOMEM;
-  int32_t *small = lists, *large = lists + nhalo, *counters = lists + 2 * nhalo;
-  PSOD_TRY(psod_fill_i32(ctx, counters, 40, 0));
-  k_plan<<<grid_for(nhalo, 4), 256, 0, st>>>(
-      c, nhalo, d_seed, d_nmem, cat->pos, cat->cell_start, plan, small, large,
-      counters, *out);
-  PSOD_LAUNCH_CHECK();
-  k_plan_lists<<<grid_for(nhalo, 256), 256, 0, st>>>(nhalo, plan, small, large,
-                                                     counters);
-  PSOD_LAUNCH_CHECK();
-  int32_t hc[4];
-  PSOD_HIP(hipMemcpyAsync(hc, counters, 16, hipMemcpyDeviceToHost, st));
-  PSOD_HIP(hipStreamSynchronize(st));
-  // halos of the large path known from the plan; the small path appends the
-  // ones it defers (their centre found) behind them
-  const int32_t nlarge1 = hc[1];
-  // Small halos, persistent (a few workgroups per CU), on the side stream:
-  // they overlap the centre search of the large halos, which touches other
-  // halos' plans and outputs.  They are queued once the search's setup has
-  // run, so that its short kernels do not wait for the small halos' CUs.
-  auto launch_small = [&]() -> int {
-    PSOD_HIP(hipEventRecord(ctx->fork, st));
-    PSOD_HIP(hipStreamWaitEvent(ctx->side, ctx->fork, 0));
-    k_sod_small<<<(unsigned)ctx->num_cus * 8, 64, 0, ctx->side>>>(
-        c, cat->cell_start, (const float4 *)cat->rec, cat->pos, cat->vel,
-        cat->ids, plan, small, large, counters, *out);
-    PSOD_LAUNCH_CHECK();
-    PSOD_HIP(hipEventRecord(ctx->join, ctx->side));
-    return PSOD_OK;
-  };
-  for (int i = 0; i < 10; i++) ctx->sod_stat[i] = 0;
-  // largest halos first (centre-search tasks and growth), so that the
-  // persistent workgroups do not finish on a giant halo picked up last
-  auto size_order = [&](int32_t n, int growth) -> int {
-    uint64_t *keys = (uint64_t *)psod_scratch(ctx, S_KEYS, (size_t)n * 8);
-    uint32_t *vals = (uint32_t *)psod_scratch(ctx, S_VALS, (size_t)n * 4);
-    if (!keys || !vals) return PSOD_E_NOMEM;
-    k_large_keys<<<grid_for(n, 256), 256, 0, st>>>(
-        c, plan, large, n, cat->cell_start, growth, keys, vals);
-    PSOD_LAUNCH_CHECK();
-    PSOD_TRY(psod_radix_sort_u64(ctx, n, keys, vals, 48));
-    PSOD_HIP(hipMemcpyAsync(large, vals, (size_t)n * 4,
-                            hipMemcpyDeviceToDevice, st));
-    return PSOD_OK;
-  };
-  // the S_MISC layout below is sized for every halo: the growth's retry
-  // list holds up to all of the large halos
+  // sized for every halo: the growth's retry list holds up to all of them
   const int32_t ncap = (int32_t)nhalo;
   int32_t *ntask = (int32_t *)psod_scratch(
       ctx, S_MISC, (size_t)(5 * ncap + 128) * 4 + (size_t)ncap * 16 + 512);
-  if (!ntask) return PSOD_E_NOMEM;
+  if (!S->plan || !lists || !ntask) return PSOD_E_NOMEM;
+  S->small = lists;
+  S->large = lists + nhalo;
+  S->counters = lists + 2 * nhalo;
+  S->ntask = ntask;
   // best S found so far per large halo (branch and bound of the search)
-  unsigned long long *hbest = (unsigned long long *)(
+  S->hbest = (unsigned long long *)(
       (((uintptr_t)(ntask + 5 * ncap + 128)) + 255) & ~(uintptr_t)255);
-  int32_t *task0 = ntask + ncap + 16;
-  int32_t *ntot = task0 + ncap + 16;
-  int32_t *retry = ntot + 16;  // positions of halos that need larger slices
-  if (nlarge1 > 0) {
-    PSOD_TRY(size_order(nlarge1, 0));
-    PSOD_TRY(centre_search(ctx, c, cat, plan, large, nlarge1, ntask, task0,
-                           ntot, hbest, launch_small));
-  } else {
-    PSOD_TRY(launch_small());
-  }
-  PSOD_HIP(hipStreamWaitEvent(st, ctx->join, 0));
-  PSOD_HIP(hipMemcpyAsync(hc, counters, 16, hipMemcpyDeviceToHost, st));
-  PSOD_HIP(hipStreamSynchronize(st));
-  const int32_t nlarge = hc[1];
-  ctx->sod_stat[0] = nhalo - nlarge;
-  ctx->sod_stat[1] = nlarge;
-  if (nlarge == 0) return PSOD_OK;
-  PSOD_TRY(size_order(nlarge, 1));
-  // one 512-thread workgroup per CU (the kernel's registers allow 8 waves)
-  const unsigned ngrow = (unsigned)min(nlarge, ctx->num_cus);
-  int32_t *hist = (int32_t *)psod_scratch(ctx, S_BINS,
-                                          (size_t)ngrow * HIST_SLICE * 4);
-  // keys per workgroup slice; PSOD_SLICE_KEYS lowers it (tests of the
-  // retry launch on small catalogues)
-  int64_t pool_cap = (int64_t)1 << 20;
-  if (const char *e = getenv("PSOD_SLICE_KEYS")) {
-    const long long v = atoll(e);
-    if (v >= 16 && v < pool_cap) pool_cap = v;
-  }
-  uint64_t *pool = (uint64_t *)psod_scratch(ctx, S_COLLECT,
-                                            (size_t)pool_cap * 8 * ngrow);
-  if (!hist || !pool) return PSOD_E_NOMEM;
-  // counters[16..17]: largest slice a failed halo needs; counters[24..31]
-  // and [32..33]: the same for a re-run launch
-  unsigned long long *needmax = (unsigned long long *)(counters + 16);
-  k_grow_large<<<ngrow, GL_T, 0, st>>>(
-      c, cat->cell_start, (const float4 *)cat->rec, cat->pos, cat->vel,
-      cat->ids, plan, large, counters, hist, pool, pool_cap, nullptr,
-      retry, needmax, *out);
-  PSOD_LAUNCH_CHECK();
-  int32_t hf[2];
-  unsigned long long hneed = 0;
-  PSOD_HIP(hipMemcpyAsync(hf, counters + 4, 8, hipMemcpyDeviceToHost, st));
-  PSOD_HIP(hipMemcpyAsync(&hneed, needmax, 8, hipMemcpyDeviceToHost, st));
-  PSOD_HIP(hipStreamSynchronize(st));
-  if (ctx->sod_stat[7] > 0) {
-    float ms = 0.0f;
-    if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]) == hipSuccess)
-      ctx->sod_stat[8] = (int64_t)((double)ms * 1e6);
-  }
-  // Further launches for the halos whose bands exceeded the slice: fewer
-  // workgroups with larger slices.  A halo re-run from its first reach may
-  // need more than its first failure showed (a larger reach later), so the
-  // slice grows at least 4x per launch.
-  int32_t *cur = retry, *nxt = retry + nlarge + 16;
-  int32_t nfail = hf[0];
+  S->task0 = ntask + ncap + 16;
+  S->ntot = S->task0 + ncap + 16;
+  S->retry = S->ntot + 16;  // positions of halos that need larger slices
+  return PSOD_OK;
+}
+
+// Launches of the growth of large halos that failed the first launch (their
+// bands exceeded the slices): fewer workgroups with larger slices.  A halo
+// re-run from its first reach may need more than its first failure showed
+// (a larger reach later), so the slice grows at least 4x per launch.
+static int grow_retries(psod_ctx *ctx, const SodCfg &c,
+                        const psod_catalogue *cat, const SodScratch &S,
+                        int32_t nlarge, int32_t ngrow, int32_t *hist,
+                        int64_t pool_cap, int32_t nfail, uint64_t hneed,
+                        int32_t *unclosed, const psod_halo_out *out) {
+  hipStream_t st = ctx->stream;
+  int32_t *cur = S.retry, *nxt = S.retry + nlarge + 16;
   int64_t cap2 = pool_cap;
   for (int pass = 0; nfail > 0; pass++) {
     const int64_t budget = (int64_t)32 << 30;  // bytes for all slices
@@ -2583,7 +2561,7 @@ static int sod_fof_run(psod_ctx *ctx, const psod_sod_params *p,
       const int32_t nshow = std::min<int32_t>(nfail, 64);
       std::vector<int32_t> pe(nshow), ph(nlarge);
       PSOD_HIP(hipMemcpy(pe.data(), cur, nshow * 4, hipMemcpyDeviceToHost));
-      PSOD_HIP(hipMemcpy(ph.data(), large, (size_t)nlarge * 4,
+      PSOD_HIP(hipMemcpy(ph.data(), S.large, (size_t)nlarge * 4,
                          hipMemcpyDeviceToHost));
       for (int32_t i = 0; i < nshow; i++) fprintf(stderr, " %d", ph[pe[i]]);
       fprintf(stderr, "\n");
@@ -2596,28 +2574,235 @@ static int sod_fof_run(psod_ctx *ctx, const psod_sod_params *p,
     if (!pool2) return PSOD_E_NOMEM;
     if (pass == 0) ctx->sod_stat[2] = nfail;
     ctx->sod_stat[3] = cap2;
-    int32_t *c2 = counters + 24;
-    unsigned long long *need2 = (unsigned long long *)(counters + 32);
+    int32_t *c2 = S.counters + 24;
+    unsigned long long *need2 = (unsigned long long *)(S.counters + 32);
     k_retry_reset<<<1, 64, 0, st>>>(c2, nfail, need2);
     PSOD_LAUNCH_CHECK();
     k_grow_large<<<ngrow2, GL_T, 0, st>>>(
         c, cat->cell_start, (const float4 *)cat->rec, cat->pos, cat->vel,
-        cat->ids, plan, large, c2, hist, pool2, cap2, cur, nxt, need2,
-        *out);
+        cat->ids, S.plan, S.large, c2, hist, pool2, cap2, cur, nxt, need2,
+        nullptr, *out);
     PSOD_LAUNCH_CHECK();
     int32_t hf2[2];
     PSOD_HIP(hipMemcpyAsync(hf2, c2 + 4, 8, hipMemcpyDeviceToHost, st));
     PSOD_HIP(hipMemcpyAsync(&hneed, need2, 8, hipMemcpyDeviceToHost, st));
     PSOD_HIP(hipStreamSynchronize(st));
     nfail = hf2[0];
-    hf[1] += hf2[1];
+    *unclosed += hf2[1];
     std::swap(cur, nxt);
   }
-  if (hf[1] > 0)
+  return PSOD_OK;
+}
+
+// keys per workgroup slice of the growth's collection pool; PSOD_SLICE_KEYS
+// lowers it (tests of the re-run launch on small catalogues)
+static int64_t growth_pool_cap() {
+  int64_t pool_cap = (int64_t)1 << 20;
+  if (const char *e = getenv("PSOD_SLICE_KEYS")) {
+    const long long v = atoll(e);
+    if (v >= 16 && v < pool_cap) pool_cap = v;
+  }
+  return pool_cap;
+}
+
+// The batched pass up to the first growth launch.  With dev set (a pass
+// that is being captured, or the eager run that sizes its buffers) nothing
+// is read back and the growth's failures are left on the device for
+// sod_finish_growth.
+static int sod_fof_queue(psod_ctx *ctx, const psod_sod_params *p,
+                         const psod_catalogue *cat, int64_t nhalo,
+                         const int32_t *d_seed, const int32_t *d_nmem,
+                         const psod_halo_out *out, bool dev, SodScratch *S,
+                         int32_t *nlarge_out) {
+  PSOD_TRY(check_sod_args(p, cat, nhalo, d_seed, out));
+  if (!out->profile)
+    return psod_set_error(PSOD_E_ARG, "sod: profile output is required");
+  if (nhalo >= (1ll << 31) - 64)
+    return psod_set_error(PSOD_E_ARG, "sod: %lld halos", (long long)nhalo);
+  SodCfg c = make_cfg(p, cat->npart);
+  const psod_tunables tun = psod_tunables_for(ctx, p->nx);
+  // the one-wave path holds at most CAP5 particles in LDS
+  c.small_max = p->small_max > 0 ? std::min(p->small_max, CAP5)
+                                 : std::min(tun.small_max, CAP5);
+  c.bound_min = tun.bound_min;
+  hipStream_t st = ctx->stream;
+  PSOD_TRY(sod_scratch(ctx, nhalo, S));
+  HaloPlan *plan = S->plan;
+  int32_t *small = S->small, *large = S->large, *counters = S->counters;
+  PSOD_TRY(psod_fill_i32(ctx, counters, 40, 0));
+  k_plan<<<grid_for(nhalo, 4), 256, 0, st>>>(
+      c, nhalo, d_seed, d_nmem, cat->pos, cat->cell_start, plan, small, large,
+      counters, *out);
+  PSOD_LAUNCH_CHECK();
+  k_plan_lists<<<grid_for(nhalo, 256), 256, 0, st>>>(nhalo, plan, small, large,
+                                                     counters);
+  PSOD_LAUNCH_CHECK();
+  // halos of the large path known from the plan; the small path appends the
+  // ones it defers (their centre found) behind them
+  int32_t nlarge1 = (int32_t)nhalo;  // a bound in the device-count mode
+  const int32_t *dn1 = nullptr;
+  if (dev) {
+    k_copy_i32<<<1, 64, 0, st>>>(counters + 1, counters + 37);
+    PSOD_LAUNCH_CHECK();
+    dn1 = counters + 37;
+  } else {
+    int32_t hc[4];
+    PSOD_HIP(hipMemcpyAsync(hc, counters, 16, hipMemcpyDeviceToHost, st));
+    PSOD_HIP(hipStreamSynchronize(st));
+    nlarge1 = hc[1];
+  }
+  // Small halos, persistent (a few workgroups per CU), on the side stream:
+  // they overlap the centre search of the large halos, which touches other
+  // halos' plans and outputs.  They are queued once the search's setup has
+  // run, so that its short kernels do not wait for the small halos' CUs.
+  auto launch_small = [&]() -> int {
+    PSOD_HIP(hipEventRecord(ctx->fork, st));
+    PSOD_HIP(hipStreamWaitEvent(ctx->side, ctx->fork, 0));
+    k_sod_small<<<(unsigned)ctx->num_cus * tun.small_wg_per_cu, 64, 0,
+                  ctx->side>>>(
+        c, cat->cell_start, (const float4 *)cat->rec, cat->pos, cat->vel,
+        cat->ids, plan, small, large, counters, *out);
+    PSOD_LAUNCH_CHECK();
+    PSOD_HIP(hipEventRecord(ctx->join, ctx->side));
+    return PSOD_OK;
+  };
+  if (!dev)
+    for (int i = 0; i < 10; i++) ctx->sod_stat[i] = 0;
+  // largest halos first (centre-search tasks and growth), so that the
+  // persistent workgroups do not finish on a giant halo picked up last
+  auto size_order = [&](int32_t n, const int32_t *dn, int growth) -> int {
+    uint64_t *keys = (uint64_t *)psod_scratch(ctx, S_KEYS, (size_t)n * 8);
+    uint32_t *vals = (uint32_t *)psod_scratch(ctx, S_VALS, (size_t)n * 4);
+    if (!keys || !vals) return PSOD_E_NOMEM;
+    k_large_keys<<<grid_for(n, 256), 256, 0, st>>>(
+        c, plan, large, n, dn, cat->cell_start, growth, keys, vals);
+    PSOD_LAUNCH_CHECK();
+    PSOD_TRY(psod_radix_sort_u64(ctx, n, keys, vals, 48, dn));
+    PSOD_HIP(hipMemcpyAsync(large, vals, (size_t)n * 4,
+                            hipMemcpyDeviceToDevice, st));
+    return PSOD_OK;
+  };
+  if (nlarge1 > 0) {
+    PSOD_TRY(size_order(nlarge1, dn1, 0));
+    PSOD_TRY(centre_search(ctx, c, cat, plan, large, nlarge1, dn1,
+                           counters + 36, S->ntask, S->task0, S->ntot,
+                           S->hbest, launch_small));
+  } else {
+    PSOD_TRY(launch_small());
+  }
+  PSOD_HIP(hipStreamWaitEvent(st, ctx->join, 0));
+  int32_t nlarge = (int32_t)nhalo;  // a bound in the device-count mode
+  const int32_t *dn = dev ? counters + 1 : nullptr;
+  if (!dev) {
+    int32_t hc[4];
+    PSOD_HIP(hipMemcpyAsync(hc, counters, 16, hipMemcpyDeviceToHost, st));
+    PSOD_HIP(hipStreamSynchronize(st));
+    nlarge = hc[1];
+    ctx->sod_stat[0] = nhalo - nlarge;
+    ctx->sod_stat[1] = nlarge;
+  }
+  *nlarge_out = nlarge;
+  if (nlarge == 0) return PSOD_OK;
+  PSOD_TRY(size_order(nlarge, dn, 1));
+  // one 512-thread workgroup per CU (the kernel's registers allow 8 waves)
+  const unsigned ngrow = (unsigned)std::min<int32_t>(nlarge, ctx->num_cus);
+  int32_t *hist = (int32_t *)psod_scratch(ctx, S_BINS,
+                                          (size_t)ngrow * HIST_SLICE * 4);
+  const int64_t pool_cap = growth_pool_cap();
+  uint64_t *pool = (uint64_t *)psod_scratch(ctx, S_COLLECT,
+                                            (size_t)pool_cap * 8 * ngrow);
+  if (!hist || !pool) return PSOD_E_NOMEM;
+  // counters[16..17]: largest slice a failed halo needs; counters[24..31]
+  // and [32..33]: the same for a re-run launch
+  unsigned long long *needmax = (unsigned long long *)(counters + 16);
+  k_grow_large<<<ngrow, GL_T, 0, st>>>(
+      c, cat->cell_start, (const float4 *)cat->rec, cat->pos, cat->vel,
+      cat->ids, plan, large, counters, hist, pool, pool_cap, nullptr,
+      S->retry, needmax, dev ? counters + 36 : nullptr, *out);
+  PSOD_LAUNCH_CHECK();
+  return PSOD_OK;
+}
+
+// After the first growth launch: reads the failures back, runs the re-run
+// launches they need and reports unclosed halos.  nlarge is the large-halo
+// count (read back here when the pass ran with device counts).
+static int sod_finish_growth(psod_ctx *ctx, const psod_sod_params *p,
+                             const psod_catalogue *cat, int64_t nhalo,
+                             const SodScratch &S, int32_t nlarge, bool dev,
+                             const psod_halo_out *out) {
+  hipStream_t st = ctx->stream;
+  int32_t hc[8];
+  unsigned long long hneed = 0;
+  PSOD_HIP(hipMemcpyAsync(hc, S.counters, 32, hipMemcpyDeviceToHost, st));
+  PSOD_HIP(hipMemcpyAsync(&hneed, S.counters + 16, 8, hipMemcpyDeviceToHost,
+                          st));
+  PSOD_HIP(hipStreamSynchronize(st));
+  if (dev) {
+    nlarge = hc[1];
+    ctx->sod_stat[0] = nhalo - nlarge;
+    ctx->sod_stat[1] = nlarge;
+  }
+  if (nlarge == 0) return PSOD_OK;
+  if (ctx->sod_stat[7] > 0 && !dev) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]) == hipSuccess)
+      ctx->sod_stat[8] = (int64_t)((double)ms * 1e6);
+  }
+  SodCfg c = make_cfg(p, cat->npart);
+  const psod_tunables tun = psod_tunables_for(ctx, p->nx);
+  c.small_max = p->small_max > 0 ? std::min(p->small_max, CAP5)
+                                 : std::min(tun.small_max, CAP5);
+  c.bound_min = tun.bound_min;
+  const int32_t ngrow = std::min<int32_t>(nlarge, ctx->num_cus);
+  int32_t *hist = (int32_t *)psod_scratch(ctx, S_BINS,
+                                          (size_t)ngrow * HIST_SLICE * 4);
+  if (!hist) return PSOD_E_NOMEM;
+  int32_t unclosed = hc[5];
+  PSOD_TRY(grow_retries(ctx, c, cat, S, nlarge, ngrow, hist,
+                        growth_pool_cap(), hc[4], hneed, &unclosed, out));
+  if (unclosed > 0)
     return psod_set_error(PSOD_E_UNCLOSED,
                           "sod: %d halos do not close within half the box",
-                          hf[1]);
+                          unclosed);
   return PSOD_OK;
+}
+
+static int sod_fof_run(psod_ctx *ctx, const psod_sod_params *p,
+                       const psod_catalogue *cat, int64_t nhalo,
+                       const int32_t *d_seed, const int32_t *d_nmem,
+                       const psod_halo_out *out) {
+  if (nhalo <= 0) return check_sod_args(p, cat, nhalo, d_seed, out);
+  SodScratch S;
+  int32_t nlarge = 0;
+  PSOD_TRY(sod_fof_queue(ctx, p, cat, nhalo, d_seed, d_nmem, out, false, &S,
+                         &nlarge));
+  return sod_finish_growth(ctx, p, cat, nhalo, S, nlarge, false, out);
+}
+
+// The device-count form of psod_sod_fof (psod_pass_graph): queues the whole
+// pass with no read-back (dev), or finishes one (reads the failures and the
+// overflow flag back, runs re-runs).  *overflow (finish only) receives the
+// flag of a task count beyond the capacity: the outputs are then invalid.
+int psod_sod_fof_dev(psod_ctx *ctx, const psod_sod_params *p,
+                     const psod_catalogue *cat, int64_t nhalo,
+                     const int32_t *d_seed, const int32_t *d_nmem,
+                     const psod_halo_out *out, bool finish,
+                     int32_t *overflow) {
+  if (nhalo <= 0) return check_sod_args(p, cat, nhalo, d_seed, out);
+  SodScratch S;
+  if (!finish) {
+    int32_t nlarge = 0;
+    return sod_fof_queue(ctx, p, cat, nhalo, d_seed, d_nmem, out, true, &S,
+                         &nlarge);
+  }
+  PSOD_TRY(sod_scratch(ctx, nhalo, &S));
+  int32_t ov = 0;
+  PSOD_HIP(hipMemcpyAsync(&ov, S.counters + 36, 4, hipMemcpyDeviceToHost,
+                          ctx->stream));
+  PSOD_HIP(hipStreamSynchronize(ctx->stream));
+  if (overflow) *overflow = ov;
+  if (ov) return PSOD_OK;
+  return sod_finish_growth(ctx, p, cat, nhalo, S, 0, true, out);
 }
 
 // Every return, failed ones included, waits for both streams: the small

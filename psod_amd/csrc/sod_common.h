@@ -21,6 +21,9 @@ struct SodCfg {
   double delta2, lr0, lr1, rmin_frac, rmax_frac, mmin, mpart;
   double rs2, eps2, tol_abs, nbar;
   float rs2f, eps2f;
+  // tunables: one-wave halo cap (particles in the 5^3 cells) and the size
+  // from which the centre search bounds its boxes first
+  int32_t small_max, bound_min;
 };
 
 // overdensity of the k closest particles at radius r, in the reference's

@@ -21,9 +21,10 @@ def host(t):
     return t.cpu().numpy()
 
 
-def check_sod(o, r, nd, min_same=0.99):
+def check_sod(o, r, nd, min_same=0.99, nx=None):
     """Batched outputs o against oracle outputs r (same halos, same seeds).
-    Returns the mask of halos with the same centre."""
+    Returns the mask of halos with the same centre.  nx (cells per side)
+    bounds the centre-of-mass shift of a boundary swap (below)."""
     assert np.all(o["status"] & 1), "every halo closes"
     same = o["centre"] == r["centre"]
     robust = r["robust_centre"] == 1
@@ -34,10 +35,17 @@ def check_sod(o, r, nd, min_same=0.99):
         cnt = o["count"][:, d] == r["count"][:, d]
         rob = r["margin"][:, d] > TOL_TIE
         assert np.all(cnt | ~same | ~rob), np.where(same & rob & ~cnt)
-        # discrete R is the FP64 distance of the same particle
+        # discrete R is the FP64 distance of the boundary particle, the
+        # (count)-th closest: bit-exact unless another particle lies within
+        # TOL_TIE of it (oracle rgap), when the split-float32 ranking may
+        # take that one; R then keeps the 1e-6 relative bar (float32
+        # distances are good to about 1e-6 cells)
         ok = same & cnt
-        np.testing.assert_allclose(o["radius"][ok, d], r["radius"][ok, d],
-                                   rtol=1e-12)
+        rg = r["rgap"][:, d] > TOL_TIE
+        np.testing.assert_allclose(o["radius"][ok & rg, d],
+                                   r["radius"][ok & rg, d], rtol=1e-12)
+        np.testing.assert_allclose(o["radius"][ok & ~rg, d],
+                                   r["radius"][ok & ~rg, d], rtol=1e-6)
         # interpolated R and M are continuous across near-ties
         np.testing.assert_allclose(o["rint"][same, d], r["rint"][same, d],
                                    rtol=1e-6)
@@ -51,7 +59,15 @@ def check_sod(o, r, nd, min_same=0.99):
     np.testing.assert_allclose(o["com_vel"][ok], r["com_vel"][ok], rtol=1e-9,
                                atol=1e-9)
     dp = np.abs(o["com_pos"][ok] - r["com_pos"][ok])
-    assert np.all(np.minimum(dp, 1 - dp) < 1e-12)
+    dp = np.minimum(dp, 1 - dp)
+    # the same particle set: the same centre of mass (sums of exact float
+    # offsets); a swap across the boundary (rgap below TOL_TIE) moves it by
+    # at most 2R/n cells
+    rg0 = r["rgap"][ok, 0] > TOL_TIE
+    assert np.all(dp[rg0] < 1e-12), dp[rg0].max()
+    if nx is not None:
+        lim = 2.0 * r["radius"][ok, 0] / np.maximum(r["count"][ok, 0], 1) / nx
+        assert np.all(dp[~rg0] <= lim[~rg0][:, None] + 1e-12)
     # acceptance (mass and centre window, main file:2118-2120)
     acc = (o["status"] & 2) != 0
     np.testing.assert_array_equal(acc[ok], r["accepted"][ok] == 1)
@@ -87,12 +103,13 @@ def subbox(pos, nx, lo, sub, vel=None):
 
 
 def oracle_subbox(orc, spos, sidx, sub, seeds, nmem, mpart, sod_kw, vel=None,
-                  centres=None):
+                  centres=None, tol_abs=None):
     """The oracle's FoF-mode SOD of the halos with full-box seeds `seeds`
     (all inside the sub-box) on the sub-box particles, with the full box's
     particle mass.  Results are translated to full-box indices (centre,
     centre_id = the generator's id = index); com_pos stays in sub-box
-    units (see com_to_full)."""
+    units (see com_to_full).  tol_abs is the full box's boundary listing
+    width in cells (the pass's default is 1e-7 nx of the full box)."""
     pos_in = np.searchsorted(sidx, seeds)
     assert np.all(sidx[pos_in] == seeds)
     cen_in = None
@@ -102,6 +119,8 @@ def oracle_subbox(orc, spos, sidx, sub, seeds, nmem, mpart, sod_kw, vel=None,
     cs, perm = orc.cell_sort(spos, sub, 8)
     fp = orc.fof_params(sub, len(spos), **sod_kw)
     fp.mpart = mpart
+    if tol_abs is not None:  # the full box's boundary width, in cells
+        fp.tol_abs = tol_abs
     r = orc.sod_fof(spos, cs, perm, 8, pos_in, nmem, fp, vel=vel, ids=sidx,
                     centres=cen_in)
     r["centre_sub"] = r["centre"].copy()

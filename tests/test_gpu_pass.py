@@ -72,6 +72,38 @@ def test_cell_sort_exact(ctx, orc, c64):
     np.testing.assert_array_equal(host(cat["lab_sorted"]), host(c64["lab"])[idx])
 
 
+def _morton9(rec):
+    """Morton key of the 1/8-cell sub-cell of each record (the order the
+    cell sort gives the records of a dense cell)."""
+    q = (np.floor((rec[:, :3] - np.floor(rec[:, :3])) * 8).astype(np.int64)
+         & 7)
+
+    def spread(v):
+        return (v & 1) | ((v & 2) << 2) | ((v & 4) << 4)
+    return spread(q[:, 0]) | (spread(q[:, 1]) << 1) | (spread(q[:, 2]) << 2)
+
+
+def test_dense_cells_in_morton_order(ctx):
+    """Cells of at least 64 records, those sorted in registers (<= 4096)
+    and those staged through a slice (> 4096), hold their records in Morton
+    order of their 1/8-cell sub-cells: the centre search culls neighbour
+    chunks by the compact boxes this gives."""
+    import psod_amd as P
+    nx = 64
+    g = P.gen_params(nx ** 3, 50, nlo=20, nhi=3000, seed=3,
+                     ncentral=120000, ccentral=8.0)
+    pos, _, lab, _, _ = ctx.generate(g, with_vel=False)
+    cat = ctx.cell_sort(pos, nx, labels=lab)
+    cs = host(cat["cell_start"]).astype(np.int64)
+    key = _morton9(host(cat["rec"]))
+    n = np.diff(cs)
+    dense = np.where(n >= 64)[0]
+    assert (n[dense] <= 4096).any() and (n[dense] > 4096).any(), n.max()
+    for c in dense:
+        k = key[cs[c]:cs[c + 1]]
+        assert np.all(np.diff(k) >= 0), (c, n[c])
+
+
 def test_cell_sort_wraps_unit_coordinate(ctx):
     import torch
     pos = torch.tensor([[1.0, 0.5, 0.25], [0.0, 0.999999, 0.5]],
