@@ -113,6 +113,20 @@ int psod_cic_deposit(psod_ctx *ctx, int64_t npart, int32_t nx,
                      const int32_t *d_cell_start, const float *d_rec,
                      int32_t ng, int32_t shift, float *d_grid);
 
+/* No counterpart in the reference.  psod_cic_deposit for callers that cannot
+ * allocate device memory: code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  Same arguments and result (bit-identical
+ * grid).  Its tile list lives in a fixed array of each device, sized for the
+ * largest valid grid (2^21 + 8256 int32, about 8.4 MB, held until the
+ * process ends; sizes beyond it return PSOD_E_CAPACITY).  It allocates
+ * nothing and does not synchronise: the grid is complete when the context's
+ * stream (psod_stream) reaches this point.  Every call on one device shares
+ * that array, so calls that overlap on one GPU (from two contexts, or two
+ * streams) race. */
+int psod_cic_deposit_noalloc(psod_ctx *ctx, int64_t npart, int32_t nx,
+                             const int32_t *d_cell_start, const float *d_rec,
+                             int32_t ng, int32_t shift, float *d_grid);
+
 /* ---- a3 (+ FoF seeds): CIC^-1 (main file:1268-1390) -----------------------
  * Interpolates the grid back to every particle.  Optional outputs:
  *   d_rho_sorted [npart] in sorted order, d_rho [npart] in original order,

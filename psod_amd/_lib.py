@@ -23,7 +23,7 @@ EXPORTS = [
     "psod_sod_fof", "psod_sod_fof_plain", "psod_label_counts",
     "psod_gen_clumps", "psod_generate", "psod_host_cic", "psod_host_cicm1",
     "psod_host_tri_cic", "psod_host_spherover", "psod_density_order_f64",
-    "psod_set_tunables", "psod_get_tunables",
+    "psod_cic_deposit_noalloc", "psod_set_tunables", "psod_get_tunables",
     "psod_autotune", "psod_load_tunables", "psod_pass_graph",
 ]
 
@@ -113,6 +113,8 @@ def load():
         "psod_sod_stats": (C.c_int, [v, P(i64), i32]),
         "psod_cell_sort": (C.c_int, [v, i64, v, v, i32, v, v, v, v]),
         "psod_cic_deposit": (C.c_int, [v, i64, i32, v, v, i32, i32, v]),
+        "psod_cic_deposit_noalloc": (C.c_int, [v, i64, i32, v, v, i32, i32,
+                                               v]),
         "psod_cic_interp": (C.c_int, [v, i64, i32, v, v, i32, i32, v, v, v,
                                       v, i64, v]),
         "psod_seed_index": (C.c_int, [v, i64, v, v]),

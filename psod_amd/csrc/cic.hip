@@ -19,6 +19,7 @@
 #include "psod_internal.h"
 
 #include <algorithm>
+#include <mutex>
 
 int psod_fill_i32(psod_ctx *c, int32_t *p, int64_t n, int32_t v);
 int psod_fill_u64(psod_ctx *c, uint64_t *p, int64_t n, uint64_t v);
@@ -459,6 +460,66 @@ extern "C" int psod_cic_deposit(psod_ctx *ctx, int64_t npart, int32_t nx,
   PSOD_TRY(deposit_queue(ctx, npart, nx, d_cell_start, d_rec, ng, shift,
                          d_grid, listed));
   return psod_finish(ctx);
+}
+
+// ---------------------------------------------------------------------------
+// The deposit for callers that cannot allocate device memory.  Its tile list
+// lives in `scratch`, a fixed array of every device the library runs on,
+// large enough for the largest valid grid (nx = 1024, ng = 2048: 2^21 tiles).
+// In host code the array's name is not a device address: psod_create
+// resolves the array of its device once (hipGetSymbolAddress, with that
+// device current) and the entry point passes only that address on.
+// ---------------------------------------------------------------------------
+constexpr int32_t STATIC_TILE_INTS = (1 << 21) + DEP_CAPD + 64;
+constexpr int PSOD_MAX_DEVICES = 64;
+static __device__ __align__(256) int32_t scratch[STATIC_TILE_INTS];
+static int32_t *g_scratch[PSOD_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+// Called by psod_create with `device` current.
+int psod_static_scratch_init(int device) {
+  if (device < 0 || device >= PSOD_MAX_DEVICES)
+    return psod_set_error(PSOD_E_ARG, "device %d beyond the %d the library "
+                          "supports", device, PSOD_MAX_DEVICES);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch[device]) return PSOD_OK;
+  void *p = nullptr;
+  PSOD_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p)
+    return psod_set_error(PSOD_E_DEVICE, "no address for the fixed tile "
+                          "list of device %d", device);
+  g_scratch[device] = (int32_t *)p;
+  return PSOD_OK;
+}
+
+extern "C" int psod_cic_deposit_noalloc(psod_ctx *ctx, int64_t npart,
+                                        int32_t nx,
+                                        const int32_t *d_cell_start,
+                                        const float *d_rec, int32_t ng,
+                                        int32_t shift, float *d_grid) {
+  PSOD_TRY(psod_use_device(ctx));
+  PSOD_TRY(check_grid_args(nx, ng, shift));
+  if (npart <= 0 || !d_cell_start || !d_rec || !d_grid)
+    return psod_set_error(PSOD_E_ARG, "psod_cic_deposit_noalloc: bad "
+                          "arguments");
+  int32_t TB, ntile, capd;
+  const int32_t nlist = deposit_tiles(nx, ng, &TB, &ntile, &capd);
+  if (nlist > STATIC_TILE_INTS)
+    return psod_set_error(PSOD_E_CAPACITY, "psod_cic_deposit_noalloc: %d "
+                          "tile-list entries exceed the fixed %d", nlist,
+                          STATIC_TILE_INTS);
+  int32_t *listed = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (ctx->device >= 0 && ctx->device < PSOD_MAX_DEVICES)
+      listed = g_scratch[ctx->device];
+  }
+  if (!listed)
+    return psod_set_error(PSOD_E_DEVICE, "psod_cic_deposit_noalloc: the "
+                          "fixed tile list of device %d is not set up",
+                          ctx->device);
+  return deposit_queue(ctx, npart, nx, d_cell_start, d_rec, ng, shift, d_grid,
+                       listed);
 }
 
 extern "C" int psod_cic_interp(psod_ctx *ctx, int64_t npart, int32_t nx,

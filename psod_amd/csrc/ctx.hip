@@ -47,6 +47,18 @@ extern "C" int psod_create(int device, psod_ctx **out) {
     free(c);
     return psod_set_error(PSOD_E_DEVICE, "hipEventCreate failed");
   }
+  // the fixed tile list of psod_cic_deposit_noalloc on this device
+  const int rs = psod_static_scratch_init(device);
+  if (rs != PSOD_OK) {
+    hipEventDestroy(c->ev[0]);
+    hipEventDestroy(c->ev[1]);
+    hipEventDestroy(c->fork);
+    hipEventDestroy(c->join);
+    hipStreamDestroy(c->side);
+    hipStreamDestroy(c->stream);
+    free(c);
+    return rs;
+  }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
     c->num_cus = prop.multiProcessorCount;

@@ -115,6 +115,10 @@ void psod_graph_release(psod_ctx *ctx);
 // pass is being captured.
 int psod_finish(psod_ctx *ctx);
 
+// Resolves the device address of the fixed array that
+// psod_cic_deposit_noalloc uses on `device` (which must be current).
+int psod_static_scratch_init(int device);
+
 // intermediates of the FoF-seeded pass (context scratch)
 struct PassBuffers {
   int32_t *cell_start;  // [nx^3 + 1]
