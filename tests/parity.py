@@ -86,6 +86,7 @@ def subbox(pos, nx, lo, sub, vel=None):
     indices, ascending; velocities or None)."""
     import torch
     assert all(0 <= l and l + sub <= nx for l in lo), (lo, sub, nx)
+    assert sub & (sub - 1) == 0, "sub-box side must be a power of two"
     inside = None
     for d in range(3):
         xs = pos[:, d] * nx

@@ -127,7 +127,11 @@ def check_giants(ctx, orc, w, k=5, min_nmem=0):
         if w["nmem_h"][h] < min_nmem:
             continue
         W = int(o["w"][h])
-        sub = 8 * ((2 * W + 16 + 7) // 8)
+        # a power of two: the sub-box coordinates (x nx - lo) / sub are
+        # then exact in float32
+        sub = 16
+        while sub < 2 * W + 16:
+            sub *= 2
         cc = np.floor(host(w["pos"][int(o["centre"][h])]) * nx).astype(int)
         lo = [int(min(max(c - sub // 2, 0), nx - sub)) for c in cc]
         if any(c - W - 2 < l or c + W + 2 >= l + sub for c, l in zip(cc, lo)):
@@ -226,7 +230,7 @@ def test_c4_reduced_plain_and_oracle(ctx, orc):
         np.testing.assert_allclose(ob["radius"], op["radius"], rtol=1e-6)
         np.testing.assert_allclose(ob["rint"], op["rint"], rtol=1e-6)
         assert check_giants(ctx, orc, w, k=1) == 1
-        n = check_subbox(orc, w, (16, 16, 16), 96, 16)
+        n = check_subbox(orc, w, (16, 16, 16), 128, 16)
         assert n > 20
     finally:
         release(w)

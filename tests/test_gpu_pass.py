@@ -89,9 +89,9 @@ def test_dense_cells_in_morton_order(ctx):
     order of their 1/8-cell sub-cells: the centre search culls neighbour
     chunks by the compact boxes this gives."""
     import psod_amd as P
-    nx = 64
-    g = P.gen_params(nx ** 3, 50, nlo=20, nhi=3000, seed=3,
-                     ncentral=120000, ccentral=8.0)
+    nx = 128
+    g = P.gen_params(nx ** 3, 200, nlo=20, nhi=3000, seed=3,
+                     ncentral=1_500_000, ccentral=10.0)
     pos, _, lab, _, _ = ctx.generate(g, with_vel=False)
     cat = ctx.cell_sort(pos, nx, labels=lab)
     cs = host(cat["cell_start"]).astype(np.int64)
