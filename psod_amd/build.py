@@ -72,9 +72,31 @@ def build_fortran() -> str | None:
     return out
 
 
+HDF5 = os.environ.get("PSOD_HDF5", "/opt/conda")
+H5LIB = os.path.join(HERE, "libpsod_h5.so")
+
+
+def build_h5() -> str | None:
+    """Compile the HDF5 halo writer (host C++, psod_amd/libpsod_h5.so) when
+    the HDF5 C library is present."""
+    inc = os.path.join(HDF5, "include", "hdf5.h")
+    if not os.path.exists(inc):
+        return None
+    cxx = os.environ.get("CXX", "g++")
+    tmp = H5LIB + ".tmp"
+    _run([cxx, "-O2", "-fPIC", "-shared", "-std=c++17", "-Wall", "-Wextra",
+          "-I", os.path.join(HDF5, "include"),
+          os.path.join(CSRC, "h5out.cpp"), "-o", tmp,
+          "-L" + os.path.join(HDF5, "lib"), "-lhdf5",
+          "-Wl,-rpath," + os.path.join(HDF5, "lib")])
+    os.replace(tmp, H5LIB)
+    return H5LIB
+
+
 def build() -> str:
     lib = build_lib()
     build_fortran()
+    build_h5()
     return lib
 
 
