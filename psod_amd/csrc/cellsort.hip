@@ -72,13 +72,17 @@ __device__ inline PartCell part_cell(const float *pos, int64_t i, int32_t nx) {
 // digit-major, so that the scan gives each (digit, tile) its first slot);
 // the workgroup then places its tile in chunks of CS_C elements: a counting
 // sort of the chunk in LDS followed by writes in digit order, so that
-// consecutive lanes write consecutive slots.  The first pass reads the
-// positions and builds the records.  The order inside a block is not kept:
-// the block sort below orders it by cell.
+// consecutive lanes write consecutive slots.  A chunk's run of one digit
+// averages CS_C / 128 = 64 elements, so its writes are mostly whole 128-byte
+// lines (runs of 16 left partial lines, which cost a read-modify-write).
+// The block ids, records and labels of a chunk pass through one LDS buffer
+// in turn.  The first pass reads the positions and builds the records.  The
+// order inside a block is not kept: the block sort below orders it by cell.
 // ---------------------------------------------------------------------------
 constexpr int32_t CS_T = 1 << 16;  // elements per tile
-constexpr int CS_C = 2048;         // elements per LDS chunk
-constexpr int CS_U = CS_C / 256;   // per thread
+constexpr int CS_TH = 1024;        // threads of the scatter
+constexpr int CS_C = 8192;         // elements per LDS chunk (128 KB of records)
+constexpr int CS_U = CS_C / CS_TH; // per thread
 constexpr int CS_DB = 7;           // digit bits per pass
 
 // tile t of a pass: bucket b, its j-th tile, elements [k0, k1)
@@ -109,7 +113,12 @@ __device__ inline void cs_load(int64_t k, const float *pos, const int32_t *lab,
     *blk = p.blk;
     *lb = lab ? lab[k] : -1;
   } else {
-    *r = sin[k];
+    // component-wise, so that the chunk's records stay in registers
+    const float4 v = sin[k];
+    r->x = v.x;
+    r->y = v.y;
+    r->z = v.z;
+    r->w = v.w;
     *blk = bin[k];
     *lb = lin ? lin[k] : -1;
   }
@@ -138,7 +147,7 @@ k_cs_hist(const float *__restrict__ pos, const int32_t *__restrict__ bin,
 }
 
 template <bool FROM_POS>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(CS_TH)
 k_cs_scatter(const float *__restrict__ pos, const int32_t *__restrict__ lab,
              const float4 *__restrict__ sin, const int32_t *__restrict__ bin,
              const int32_t *__restrict__ lin, int32_t nx, int shift, int dbits,
@@ -147,28 +156,29 @@ k_cs_scatter(const float *__restrict__ pos, const int32_t *__restrict__ lab,
              const int32_t *__restrict__ tmap, const int32_t *__restrict__ O,
              float4 *__restrict__ sout, int32_t *__restrict__ bout,
              int32_t *__restrict__ lout) {
-  __shared__ float4 sr[CS_C];
-  __shared__ int32_t sbk[CS_C], slb[CS_C];
+  __shared__ float4 ex[CS_C];  // the chunk in digit order, one array at a time
   __shared__ int32_t gfill[1 << CS_DB], lcnt[1 << CS_DB], lst[1 << CS_DB];
+  int32_t *exi = reinterpret_cast<int32_t *>(ex);
   int32_t b, j, k0, k1;
   if (!cs_tile(blockIdx.x, tb, tmap, bs, nbk, &b, &j, &k0, &k1)) return;
   const int32_t nd = 1 << dbits;
   const int32_t *reg = O + (int64_t)tb[b] * nd;
-  for (int d = threadIdx.x; d < nd; d += 256) gfill[d] = reg[(int64_t)d * ntb[b] + j];
+  for (int d = threadIdx.x; d < nd; d += CS_TH)
+    gfill[d] = reg[(int64_t)d * ntb[b] + j];
   for (int32_t c0 = k0; c0 < k1; c0 += CS_C) {
     const int32_t cn = min(CS_C, k1 - c0);
-    for (int d = threadIdx.x; d < nd; d += 256) lcnt[d] = 0;
+    for (int d = threadIdx.x; d < nd; d += CS_TH) lcnt[d] = 0;
     __syncthreads();
     float4 rr[CS_U];
-    int32_t bb[CS_U], ll[CS_U], rk[CS_U];
+    int32_t bb[CS_U], ll[CS_U], slot[CS_U], dst[CS_U];
 #pragma unroll
     for (int u = 0; u < CS_U; u++) {
-      const int32_t i = threadIdx.x + 256 * u;
-      rk[u] = 0;
+      const int32_t i = threadIdx.x + CS_TH * u;
+      slot[u] = 0;
       if (i < cn) {
         cs_load<FROM_POS>(c0 + i, pos, lab, sin, bin, lin, nx, &rr[u], &bb[u],
                           &ll[u]);
-        rk[u] = atomicAdd(&lcnt[(bb[u] >> shift) & (nd - 1)], 1);
+        slot[u] = atomicAdd(&lcnt[(bb[u] >> shift) & (nd - 1)], 1);
       }
     }
     __syncthreads();
@@ -180,26 +190,49 @@ k_cs_scatter(const float *__restrict__ pos, const int32_t *__restrict__ lab,
       if (d0 + 1 < nd) lst[d0 + 1] = inc - v1;
     }
     __syncthreads();
+    // block ids to their slots in digit order; each thread then takes the
+    // slots i = threadIdx.x + CS_TH * u and finds their destinations
+#pragma unroll
+    for (int u = 0; u < CS_U; u++)
+      if (threadIdx.x + CS_TH * u < cn) {
+        slot[u] += lst[(bb[u] >> shift) & (nd - 1)];
+        exi[slot[u]] = bb[u];
+      }
+    __syncthreads();
 #pragma unroll
     for (int u = 0; u < CS_U; u++) {
-      const int32_t i = threadIdx.x + 256 * u;
+      const int32_t i = threadIdx.x + CS_TH * u;
       if (i < cn) {
-        const int32_t p = lst[(bb[u] >> shift) & (nd - 1)] + rk[u];
-        sr[p] = rr[u];
-        sbk[p] = bb[u];
-        slb[p] = ll[u];
+        const int32_t bk = exi[i];
+        const int32_t d = (bk >> shift) & (nd - 1);
+        dst[u] = gfill[d] + (i - lst[d]);
+        bout[dst[u]] = bk;
       }
     }
     __syncthreads();
-    for (int32_t i = threadIdx.x; i < cn; i += 256) {
-      const int32_t d = (sbk[i] >> shift) & (nd - 1);
-      const int32_t dst = gfill[d] + (i - lst[d]);
-      sout[dst] = sr[i];
-      bout[dst] = sbk[i];
-      if (lout) lout[dst] = slb[i];
+#pragma unroll
+    for (int u = 0; u < CS_U; u++)
+      if (threadIdx.x + CS_TH * u < cn) ex[slot[u]] = rr[u];
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < CS_U; u++) {
+      const int32_t i = threadIdx.x + CS_TH * u;
+      if (i < cn) sout[dst[u]] = ex[i];
+    }
+    if (lout) {
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < CS_U; u++)
+        if (threadIdx.x + CS_TH * u < cn) exi[slot[u]] = ll[u];
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < CS_U; u++) {
+        const int32_t i = threadIdx.x + CS_TH * u;
+        if (i < cn) lout[dst[u]] = exi[i];
+      }
     }
     __syncthreads();
-    for (int d = threadIdx.x; d < nd; d += 256) gfill[d] += lcnt[d];
+    for (int d = threadIdx.x; d < nd; d += CS_TH) gfill[d] += lcnt[d];
     __syncthreads();
   }
 }
@@ -537,11 +570,11 @@ extern "C" int psod_cell_sort(psod_ctx *ctx, int64_t npart, const float *d_pos,
     PSOD_LAUNCH_CHECK();
     PSOD_TRY(psod_scan_i32(ctx, H, O, mt * nd, nullptr));
     if (p == 0) {
-      k_cs_scatter<true><<<(unsigned)mt, 256, 0, st>>>(
+      k_cs_scatter<true><<<(unsigned)mt, CS_TH, 0, st>>>(
           d_pos, d_labels, nullptr, nullptr, nullptr, nx, shift, dbits, nbk,
           bs, ntb, tb, tmap, O, recb[dst], blkb[dst], labb[dst]);
     } else {
-      k_cs_scatter<false><<<(unsigned)mt, 256, 0, st>>>(
+      k_cs_scatter<false><<<(unsigned)mt, CS_TH, 0, st>>>(
           nullptr, nullptr, recb[src], blkb[src], labb[src], nx, shift, dbits,
           nbk, bs, ntb, tb, tmap, O, recb[dst], blkb[dst], labb[dst]);
     }
