@@ -52,6 +52,9 @@ def parse():
                          "replays")
     ap.add_argument("--tune", action="store_true",
                     help="run the autotuner on the catalogue first")
+    ap.add_argument("--giant-min", type=int, default=0,
+                    help="growth cube particles from which a halo is grown by "
+                         "the staged launches (0: the context's tunable)")
     ap.add_argument("--tune-file", default=None,
                     help="tuning file the autotuner appends to")
     ap.add_argument("--no-plain", action="store_true")
@@ -268,6 +271,8 @@ def main():
     grid = torch.empty((ng, ng, ng), dtype=torch.float32, device="cuda:0")
     out = ctx.alloc_halo_out(nh, nbins)
     tuned = None
+    if args.giant_min > 0:
+        ctx.set_tunables(giant_min=args.giant_min)
     if args.tune:
         t0 = time.perf_counter()
         chosen, dep_ms, sod_ms = ctx.autotune(p, pos, lab, nh, vel=vel,
@@ -473,7 +478,8 @@ def main():
         "roofline": roofline,
         "centre_search": centre,
         "sod_paths": dict(wave=cs["wave"], workgroup=cs["workgroup"],
-                          rerun=cs["retried"]),
+                          rerun=cs["retried"], giants_staged=cs["giants"],
+                          giant_cycles=cs["giant_cycles"]),
         "plain_fp64": plain,
         "cpu_baseline": cpu,
         "errors_vs_fp64_oracle": errors,

@@ -54,7 +54,7 @@ int psod_synchronize(psod_ctx *ctx);
 void *psod_stream(psod_ctx *ctx);
 /* bytes of device scratch the context currently holds                       */
 int64_t psod_scratch_bytes(psod_ctx *ctx);
-/* counts of the last psod_sod_fof call, first min(n, 10) of: out[0] halos
+/* counts of the last psod_sod_fof call, first min(n, 12) of: out[0] halos
  * grown by one wave, out[1] halos grown by one workgroup, out[2] of these
  * re-run with larger collection slices, out[3] the keys per slice of that
  * re-run (0 if none), out[4] centre-search units summed to the end (pilot
@@ -62,7 +62,8 @@ int64_t psod_scratch_bytes(psod_ctx *ctx);
  * upper bound, out[6] candidate-neighbour pair tests, out[7] centre-search
  * tasks in all, out[8] centre-search time in ns, out[9] items abandoned
  * part way (exact sums plus the bound of the remaining neighbour cells
- * below the best S).
+ * below the best S), out[10] giant halos grown by the staged launches,
+ * out[11] their cycles (a cycle per reach tried).
  * No reference counterpart: diagnostics of the execution paths. */
 int psod_sod_stats(psod_ctx *ctx, int64_t *out, int32_t n);
 
@@ -75,9 +76,12 @@ int psod_sod_stats(psod_ctx *ctx, int64_t *out, int32_t n);
  *                    bounds its boxes of 16 candidates before summing [2048]
  *   dense_tile       own particles from which a CIC tile is dispatched first
  *                    [16384]
- *   small_wg_per_cu  one-wavefront halo workgroups per CU [8, at most 32]   */
+ *   small_wg_per_cu  one-wavefront halo workgroups per CU [8, at most 32]
+ *   giant_min        particles in a growth cube from which the halo is grown
+ *                    by the staged launches, each of its walks spread over
+ *                    the whole GPU [2097152] */
 typedef struct {
-  int32_t small_max, bound_min, dense_tile, small_wg_per_cu;
+  int32_t small_max, bound_min, dense_tile, small_wg_per_cu, giant_min;
 } psod_tunables;
 
 /* Sets / reads the context's tunables (those of psod_set_tunables override
@@ -255,7 +259,7 @@ int psod_pass(psod_ctx *ctx, const psod_sod_params *p, int64_t npart,
  * batched SOD, `reps` runs each, the fastest counts).  The choice becomes
  * the context's tunables for grids of this nx; with cache_file not NULL it
  * is also appended to that file as one line
- *   psod-tune <device name> <CUs> <nx> <npart> <nhalo> <4 tunables> <ms>
+ *   psod-tune <device name> <CUs> <nx> <npart> <nhalo> <5 tunables> <ms>
  * which psod_load_tunables reads back.  *chosen (may be NULL) receives it;
  * stage_ms (may be NULL) receives [best deposit ms, best SOD ms]. */
 int psod_autotune(psod_ctx *ctx, const psod_sod_params *p, int64_t npart,

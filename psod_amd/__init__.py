@@ -267,10 +267,11 @@ class Context:
 
     # ---- tunables ----------------------------------------------------------
     def set_tunables(self, small_max=0, bound_min=0, dense_tile=0,
-                     small_wg_per_cu=0):
+                     small_wg_per_cu=0, giant_min=0):
         """psod_set_tunables: 0 leaves a field to the tuning file or the
         built-in default."""
-        t = _lib.Tunables(small_max, bound_min, dense_tile, small_wg_per_cu)
+        t = _lib.Tunables(small_max, bound_min, dense_tile, small_wg_per_cu,
+                          giant_min)
         _lib.check(_L.psod_set_tunables(self.h, C.byref(t)))
 
     def tunables(self, nx):
@@ -323,13 +324,14 @@ class Context:
         large halos: its tasks (64 candidates of one cell) in all, units
         evaluated exactly (pilot tasks and groups of 8 candidates), units
         skipped by their bound, groups abandoned part-way, pair tests and
-        the search's time."""
-        a = (C.c_int64 * 10)()
-        _lib.check(_L.psod_sod_stats(self.h, a, 10))
+        the search's time; giant halos grown by the staged launches and
+        their cycles."""
+        a = (C.c_int64 * 12)()
+        _lib.check(_L.psod_sod_stats(self.h, a, 12))
         return dict(wave=a[0], workgroup=a[1], retried=a[2], retry_slice=a[3],
                     centre_tasks=a[4], centre_skipped=a[5], pair_tests=a[6],
                     centre_tasks_all=a[7], centre_ms=a[8] * 1e-6,
-                    centre_aborted=a[9])
+                    centre_aborted=a[9], giants=a[10], giant_cycles=a[11])
 
     # ---- host drop-ins ---------------------------------------------------
     def host_cic(self, pos, ng, mass=None, xxmin=0.0):

@@ -75,7 +75,8 @@ class GenParams(C.Structure):
 
 class Tunables(C.Structure):
     _fields_ = [("small_max", C.c_int32), ("bound_min", C.c_int32),
-                ("dense_tile", C.c_int32), ("small_wg_per_cu", C.c_int32)]
+                ("dense_tile", C.c_int32), ("small_wg_per_cu", C.c_int32),
+                ("giant_min", C.c_int32)]
 
 
 class GreedyParams(C.Structure):

@@ -68,6 +68,10 @@ enum ScratchSlot {
   S_PASS_HALO,
   S_PASS_GRID,
   S_TUNE,      // autotuner: per-halo outputs
+  S_GIANT,     // SOD: list and states of the giant halos (staged growth)
+  S_GIANT_H,   // SOD: their histogram slices
+  S_GIANT_P,   // SOD: their collection slices
+  S_GIANT_SEG, // SOD: per-segment counts of their cubes, prefix sums
   S_NSLOT
 };
 
@@ -80,7 +84,8 @@ struct TuneEntry {
 
 // built-in defaults of the tunables
 constexpr int32_t TUNE_SMALL_MAX = 1024, TUNE_BOUND_MIN = 2048,
-                  TUNE_DENSE_TILE = 1 << 14, TUNE_SMALL_WG = 8;
+                  TUNE_DENSE_TILE = 1 << 14, TUNE_SMALL_WG = 8,
+                  TUNE_GIANT_MIN = 1 << 21;
 
 struct psod_ctx {
   int device;
@@ -92,7 +97,7 @@ struct psod_ctx {
   void *slot[S_NSLOT];
   size_t cap[S_NSLOT];
   int num_cus;
-  int64_t sod_stat[10];  // see psod_sod_stats
+  int64_t sod_stat[12];  // see psod_sod_stats
   hipEvent_t ev[2];     // brackets the centre search (sod_stat[8])
   // second stream (default priority) for work that overlaps the main one,
   // and the events that fork to it and join from it

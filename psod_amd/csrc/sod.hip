@@ -66,6 +66,7 @@ SodCfg make_cfg(const psod_sod_params *p, int64_t npart) {
   c.eps2f = (float)c.eps2;
   c.tol_abs = p->tol_abs;
   c.nbar = (double)npart / ((double)p->nx * p->nx * p->nx);
+  c.giant_min = INT64_MAX;  // no staged growth unless the pass sets it
   return c;
 }
 
@@ -1596,17 +1597,23 @@ __device__ inline void cube_seg(const SodCfg &c, const CellRef &cr,
   *ncx = xb - xa + 1;
 }
 
-// particles in the cells at offsets [-W, W]^3 (cell counts only)
+// particles in the cells at offsets [-W, W]^3 (cell counts only), of the
+// segments q0 <= q < q1; with segcnt, segment q's count to segcnt[q]
 __device__ inline int64_t cube_count(const SodCfg &c, GrowShared &G,
                                      const int32_t *cell_start,
-                                     const CellRef &cr, int W) {
+                                     const CellRef &cr, int W, int64_t q0 = 0,
+                                     int64_t q1 = INT64_MAX,
+                                     int32_t *segcnt = nullptr) {
   const CubeSegs g = cube_segs(c, cr, W);
+  const int64_t qe = min(q1, g.nseg);
   int64_t n = 0;
-  for (int64_t q = threadIdx.x; q < g.nseg; q += GL_T) {
+  for (int64_t q = q0 + threadIdx.x; q < qe; q += GL_T) {
     int32_t bx, gy, gz, ka, ncx;
     int dj, dk;
     cube_seg(c, cr, g, W, (int32_t)q, &bx, &dj, &dk, &gy, &gz, &ka, &ncx);
-    n += cell_start[ka + ncx] - cell_start[ka];
+    const int32_t m = cell_start[ka + ncx] - cell_start[ka];
+    if (segcnt) segcnt[q] = m;
+    n += m;
   }
   return block_sum_i64(n, G);
 }
@@ -1658,16 +1665,19 @@ __device__ inline void bin_band(int32_t b0, int32_t b1, int32_t nb,
 // Calls f(d2, idx, rec, di, dj, dk, lc) for the particles of the cells at
 // offsets [-W, W]^3 from the reference cell, skipping the row segments that
 // cannot hold a particle with d2 in [lo2, hi2] (f sees all the others).
+// Segments q0 <= q < q1 only.
 template <typename F>
 __device__ inline void cube_walk(const SodCfg &c, GrowShared &G,
                                  const int32_t *cell_start, const float4 *rec,
                                  const CellRef &cr, int W, float lo2,
-                                 float hi2, F &&f) {
+                                 float hi2, F &&f, int64_t q0 = 0,
+                                 int64_t q1 = INT64_MAX) {
   const CubeSegs g = cube_segs(c, cr, W);
-  for (int64_t base = 0; base < g.nseg; base += GL_T) {
+  const int64_t qe = min(q1, g.nseg);
+  for (int64_t base = q0; base < qe; base += GL_T) {
     const int64_t q = base + threadIdx.x;
     int32_t cnt = 0, st = 0, sdi = 0, spk = 0;
-    if (q < g.nseg) {
+    if (q < qe) {
       int32_t bx, gy, gz, ka, ncx;
       int dj, dk;
       cube_seg(c, cr, g, W, (int32_t)q, &bx, &dj, &dk, &gy, &gz, &ka, &ncx);
@@ -1698,7 +1708,7 @@ __device__ inline void cube_walk(const SodCfg &c, GrowShared &G,
     G.spk[threadIdx.x] = spk;
     if (threadIdx.x == 0) G.pre[GL_T] = all;
     __syncthreads();
-    const int nq = (int)min((int64_t)GL_T, g.nseg - base);
+    const int nq = (int)min((int64_t)GL_T, qe - base);
     // CW_U particles per thread per step: their searches and loads are
     // independent, so their latencies overlap
     for (int32_t j0 = threadIdx.x; j0 < all; j0 += CW_U * GL_T) {
@@ -1778,12 +1788,43 @@ __device__ void block_sort_pool(uint64_t *pool, int32_t n, GrowShared &G) {
     __syncthreads();
     return;
   }
+  // The bitonic network over np = next_pow2(n) keys (padding ~0 sorts
+  // last).  Its steps (k, j) with j < SORT_CAP pair keys of one block of
+  // SORT_CAP; they run block by block in LDS, one load and store of the
+  // block per k.  Only the steps with j >= SORT_CAP pass through global
+  // memory: for np = 2^20, 36 passes instead of 210.
   const int np = next_pow2(n);
   for (int i = n + threadIdx.x; i < np; i += GL_T) pool[i] = ~0ull;
   __threadfence_block();
   __syncthreads();
-  for (int k = 2; k <= np; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
+  // steps k in [k0, k1], j < SORT_CAP, of every block; a pair's direction
+  // is that of the whole network ((i & k) == 0: ascending)
+  auto blocks = [&](int k0, int k1) {
+    for (int base = 0; base < np; base += SORT_CAP) {
+      for (int i = threadIdx.x; i < SORT_CAP; i += GL_T)
+        G.skey[i] = pool[base + i];
+      __syncthreads();
+      for (int k = k0; k <= k1; k <<= 1)
+        for (int j = min(k >> 1, SORT_CAP >> 1); j > 0; j >>= 1) {
+          for (int i = threadIdx.x; i < SORT_CAP; i += GL_T) {
+            const int p = i ^ j;
+            if (p > i) {
+              const uint64_t x = G.skey[i], y = G.skey[p];
+              const bool up = ((base + i) & k) == 0;
+              if ((x > y) == up) { G.skey[i] = y; G.skey[p] = x; }
+            }
+          }
+          __syncthreads();
+        }
+      for (int i = threadIdx.x; i < SORT_CAP; i += GL_T)
+        pool[base + i] = G.skey[i];
+      __threadfence_block();
+      __syncthreads();
+    }
+  };
+  blocks(2, SORT_CAP);
+  for (int k = 2 * SORT_CAP; k <= np; k <<= 1) {
+    for (int j = k >> 1; j >= SORT_CAP; j >>= 1) {
       for (int i = threadIdx.x; i < np; i += GL_T) {
         const int p = i ^ j;
         if (p > i) {
@@ -1795,6 +1836,665 @@ __device__ void block_sort_pool(uint64_t *pool, int32_t n, GrowShared &G) {
       __threadfence_block();
       __syncthreads();
     }
+    blocks(k, k);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Growth of one large halo, in phases:
+//   begin    size the radial histogram of the cube of reach W, or close the
+//            halo as unclosed when the cube exceeds the box;
+//   walk 1   the radial histogram (bins uniform in r^3);
+//   bins     its scan, the bins that may hold each threshold's first
+//            crossing and the ranges of bins to collect;
+//   walk 2   the keys (d2, index) of those ranges;
+//   cross    their sort, the exact crossings, the second threshold's ranges
+//            and the profile radii;
+//   walk 3   profile counts, boundary count, centre-of-mass sums and the
+//            keys of the second threshold's ranges;
+//   finish   the outputs.
+// bins and cross send the halo back to begin with a larger reach when the
+// crossing lies beyond the cube, or fail it when its bands exceed the
+// collection slice (a re-run launch then grows it with larger slices).
+// k_grow_large runs the phases of each halo in one workgroup.  A giant halo
+// (cube of at least c.giant_min particles) is left to the staged launches
+// k_giant_*: one workgroup per giant halo for begin, bins, cross and
+// finish, every workgroup of the GPU for its walks, each phase a launch of
+// its own with the halo's state kept in global memory between launches.
+// ---------------------------------------------------------------------------
+enum : int32_t {
+  GS_BEGIN = 0,
+  GS_HIST = 1,
+  GS_COLLECT = 2,
+  GS_PROFILE = 3,
+  GS_DONE = 4,
+  GS_FAILED = 5,
+  GS_COUNT = 6  // staged path: the cube's particles counted by a walk
+};
+
+constexpr int64_t CH_SEG = 4 * GL_T;  // segments per chunk of the count walk
+constexpr int64_t CH_PART = 1 << 15;  // particles per chunk of walks 1-3
+
+// a halo's growth state between phases (scalars, kept in registers; its
+// HaloResult and scaled centre are held apart: they are indexed at run time
+// and live in memory)
+struct GrowVars {
+  int64_t h;          // halo
+  int32_t e, centre;  // its position in the large list; centre particle
+  CellRef cr;
+  int32_t W, nb;      // reach (cells); radial bins
+  float scale;        // bins per unit of r^3 (cells^3)
+  float lo2, hi2;     // distance band of the next walk
+  int64_t ncube;      // particles of the cube of reach W
+  int64_t k0s;        // rank of the first crossing of delta[0]
+  uint64_t keyR;      // key of rank k0s - 1
+  double R0, rlast;   // R_delta[0]; outer profile radius
+  int32_t ns2;        // ranges of the second threshold
+};
+
+// sums of walk 3
+struct ProfileSums {
+  double sp[3], sv[3];
+  int64_t nbd;
+};
+
+__device__ inline void grow_start(const SodCfg &c, const float *pos,
+                                  const int64_t *ids, const HaloPlan *plan,
+                                  const int32_t *large, int32_t e,
+                                  GrowVars &V, HaloResult &res,
+                                  double *xcs) {
+  V.e = e;
+  V.h = large[e];
+  const HaloPlan pl = plan[V.h];
+  V.centre = pl.centre;
+  V.cr = ref_of_particle(pos, V.centre, c.nx);
+  centre_scaled(pos, V.centre, c.nx, xcs);
+  zero_result(res);
+  res.centre = V.centre;
+  res.centre_id = ids ? ids[V.centre] : (int64_t)V.centre;
+  V.W = pl.w0 < 2 ? 2 : pl.w0;
+  V.nb = 0;
+  V.scale = 0.0f;
+  V.lo2 = -1.0f;
+  V.hi2 = -2.0f;
+  V.ncube = 0;
+  V.k0s = 0;
+  V.keyR = 0;
+  V.R0 = 0.0;
+  V.rlast = 0.0;
+  V.ns2 = 0;
+}
+
+__device__ inline int64_t chunks_of(int64_t nseg) {
+  return (nseg + CH_SEG - 1) / CH_SEG;
+}
+
+// false, with the halo closed as unclosed, when the cube of reach W
+// exceeds the box
+__device__ inline bool grow_reach(const SodCfg &c, const GrowVars &V,
+                                  HaloResult &res) {
+  if (2 * V.W + 1 > c.nx) {
+    res.status = PSOD_H_UNCLOSED | PSOD_H_LARGE;
+    res.w = V.W;
+    return false;
+  }
+  return true;
+}
+
+// the radial histogram of a cube of ncube particles and the band of walk 1
+__device__ inline void grow_size(GrowVars &V, int64_t ncube) {
+  const float W2 = (float)V.W * (float)V.W;
+  // bins uniform in r^3, about 4 particles of the cube per bin
+  V.ncube = ncube;
+  int32_t nb = HIST_MIN;
+  while ((int64_t)nb * 4 < V.ncube && nb < HIST_MAX) nb <<= 1;
+  V.nb = nb;
+  V.scale = (float)nb / ((float)V.W * (float)V.W * (float)V.W);
+  // every particle of the cube is gathered; those of d2 < W2 are binned
+  V.lo2 = -1.0f;
+  V.hi2 = W2 * (1.0f + 1e-4f) + 1e-3f;
+}
+
+// begin: GS_HIST, or GS_DONE for a halo that does not close in the box
+__device__ inline int32_t grow_begin(const SodCfg &c, GrowShared &G,
+                                     const int32_t *cell_start, GrowVars &V,
+                                     HaloResult &res) {
+  if (!grow_reach(c, V, res)) return GS_DONE;
+  const CellRef cr = V.cr;
+  grow_size(V, cube_count(c, G, cell_start, cr, V.W));
+  return GS_HIST;
+}
+
+// walk 1 over the segments [q0, q1) of the cube: the particles of d2 < W^2
+// to their radial bins, the first HIST_LDS bins counted in lh
+__device__ inline void walk_hist(const SodCfg &c, GrowShared &G, int32_t *lh,
+                                 int32_t *hist, const int32_t *cell_start,
+                                 const float4 *rec, const GrowVars &V,
+                                 int64_t q0 = 0, int64_t q1 = INT64_MAX) {
+  const float W2 = (float)V.W * (float)V.W;
+  const float scale = V.scale;
+  const int32_t nb = V.nb;
+  const CellRef cr = V.cr;
+  cube_walk(c, G, cell_start, rec, cr, V.W, V.lo2, V.hi2,
+            [&](float d2, int32_t, const float4 &, int, int, int,
+                const int32_t *) {
+              if (d2 < W2) {
+                const int32_t b = rbin(d2, scale, nb);
+                // scopes differ, so that the two adds stay apart: merged
+                // into one add on a selected pointer they become a flat
+                // atomic, much slower on LDS
+                if (b < HIST_LDS)
+                  __hip_atomic_fetch_add(&lh[b], 1, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+                else
+                  __hip_atomic_fetch_add(&hist[b], 1, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+              }
+            },
+            q0, q1);
+}
+
+// walk 2: the keys of the particles whose bin lies in one of the ranges
+// [G.rs[r], G.re[r]] (r < nr) to pool[G.off[r] + fill[r]++]
+__device__ inline void walk_collect(const SodCfg &c, GrowShared &G,
+                                    const int32_t *cell_start,
+                                    const float4 *rec, const GrowVars &V,
+                                    int32_t nr, int32_t *fill, uint64_t *pool,
+                                    int64_t q0 = 0, int64_t q1 = INT64_MAX) {
+  const float W2 = (float)V.W * (float)V.W;
+  const float scale = V.scale;
+  const int32_t nb = V.nb;
+  const CellRef cr = V.cr;
+  cube_walk(c, G, cell_start, rec, cr, V.W, V.lo2, V.hi2,
+            [&](float d2, int32_t idx, const float4 &, int, int, int,
+                const int32_t *) {
+              if (!(d2 < W2)) return;
+              const int32_t b = rbin(d2, scale, nb);
+              for (int r = 0; r < nr; r++)
+                if (b >= G.rs[r] && b <= G.re[r]) {
+                  const int32_t s = atomicAdd(&fill[r], 1);
+                  pool[G.off[r] + s] = d2key(d2, idx);
+                }
+            },
+            q0, q1);
+}
+
+// walk 3: profile counts per wave in G.pcnt (radii G.prad), the boundary
+// count, the centre-of-mass sums of the keys up to keyR (offsets from the
+// reference point, velocities when flagv), and the keys of the second
+// threshold's ranges G.rs[R2 + j] to pool[G.off[R2 + j] + fill[R2 + j]++]
+__device__ inline void walk_profile(const SodCfg &c, GrowShared &G,
+                                    const int32_t *cell_start,
+                                    const float4 *rec, const float *vel,
+                                    const GrowVars &V, int flagv,
+                                    int32_t *fill, uint64_t *pool,
+                                    ProfileSums &S, int64_t q0 = 0,
+                                    int64_t q1 = INT64_MAX) {
+  const float W2 = (float)V.W * (float)V.W;
+  const float scale = V.scale;
+  const int32_t nb = V.nb, ns2 = V.ns2;
+  const double R0 = V.R0, rlast = V.rlast;
+  const uint64_t keyR = V.keyR;
+  const CellRef cr = V.cr;
+  // local sums (registers), added to S at the end
+  double sp[3] = {0.0, 0.0, 0.0}, sv[3] = {0.0, 0.0, 0.0};
+  int64_t nbd = 0;
+  cube_walk(c, G, cell_start, rec, cr, V.W, V.lo2, V.hi2,
+            [&](float d2, int32_t idx, const float4 &r, int di, int dj,
+                int dk, const int32_t *lc) {
+              if (!(d2 < W2)) return;
+              const double rr = (double)sqrtf(d2);
+              if (rr < rlast) {
+                int lo = 0, hi = c.nbins - 1;  // first j with rr < prad[j]
+                while (lo < hi) {
+                  const int mid = (lo + hi) >> 1;
+                  if (rr < G.prad[mid]) hi = mid; else lo = mid + 1;
+                }
+                atomicAdd(&G.pcnt[threadIdx.x >> 6][lo], 1);
+              }
+              if (fabs(rr - R0) <= c.tol_abs) nbd++;
+              if (d2key(d2, idx) <= keyR) {
+                sp[0] += rel1d(r.x, lc[0], di, (double)cr.f[0]);
+                sp[1] += rel1d(r.y, lc[1], dj, (double)cr.f[1]);
+                sp[2] += rel1d(r.z, lc[2], dk, (double)cr.f[2]);
+                if (flagv)
+                  for (int d = 0; d < 3; d++)
+                    sv[d] += (double)vel[3 * (int64_t)idx + d];
+              }
+              if (ns2 > 0) {
+                const int32_t b = rbin(d2, scale, nb);
+                for (int j = 0; j < ns2; j++)
+                  if (b >= G.rs[R2 + j] && b <= G.re[R2 + j]) {
+                    const int32_t s = atomicAdd(&fill[R2 + j], 1);
+                    pool[G.off[R2 + j] + s] = d2key(d2, idx);
+                  }
+              }
+            },
+            q0, q1);
+  S.nbd += nbd;
+  for (int d = 0; d < 3; d++) {
+    S.sp[d] += sp[d];
+    S.sv[d] += sv[d];
+  }
+}
+
+// bins, on the histogram of walk 1 (hist[0..nb), counted): GS_COLLECT,
+// GS_BEGIN (reach doubled) or GS_FAILED (G.need slots beyond pool_cap).
+// lh stages the scan.
+__device__ inline int32_t grow_bins(const SodCfg &c, GrowShared &G,
+                                    int32_t *lh, int32_t *hist, GrowVars &V,
+                                    int64_t pool_cap) {
+  const int32_t nb = V.nb;
+  const float W2 = (float)V.W * (float)V.W;
+  // exclusive scan in place: hist[b] = #(bin < b), hist[nb] = total.
+  // Rounds of SCAN_ROUND bins, read coalesced into LDS; thread t scans
+  // the SCAN_ROUND / GL_T consecutive bins from t * SCAN_ROUND / GL_T.
+  {
+    constexpr int PER = SCAN_ROUND / GL_T;
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    int32_t carry = 0;
+    for (int32_t base = 0; base < nb; base += SCAN_ROUND) {
+      for (int i = threadIdx.x; i < SCAN_ROUND; i += GL_T)
+        lh[i] = ldcg(&hist[base + i]);
+      __syncthreads();
+      int32_t v[PER], sum = 0;
+#pragma unroll
+      for (int i = 0; i < PER; i++) {
+        v[i] = lh[threadIdx.x * PER + i];
+        sum += v[i];
+      }
+      int32_t tot;
+      const int32_t ex = wave_excl(sum, &tot);
+      if (l == 0) G.red_i[w] = tot;
+      __syncthreads();
+      int32_t run = carry + ex, all = 0;
+      for (int i = 0; i < GL_T / 64; i++) {
+        if (i < w) run += G.red_i[i];
+        all += G.red_i[i];
+      }
+#pragma unroll
+      for (int i = 0; i < PER; i++) {
+        lh[threadIdx.x * PER + i] = run;
+        run += v[i];
+      }
+      __syncthreads();
+      for (int i = threadIdx.x; i < SCAN_ROUND; i += GL_T)
+        hist[base + i] = lh[i];
+      carry += all;
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) hist[nb] = carry;
+    __threadfence();
+    __syncthreads();
+  }
+  const int32_t *C = hist;
+  const double Wd = (double)V.W, W3 = Wd * Wd * Wd;
+  // ---- bins that may hold each threshold's first crossing --------------
+  // A bin is possible for delta when the smallest admissible rank k in it
+  // could have od_k <= delta at the bin's outer edge, and definite when
+  // it must at the inner edge; the crossing lies in a possible bin no
+  // later than the first definite one.  The possible bins are collected
+  // as up to SEGCAP segments, each extended down to the previous
+  // non-empty bin, which holds rank k-1 when k opens the segment.
+  bool ok = true;
+  auto bin_k = [&](int32_t b, int64_t *kmin) {
+    const int32_t cb = ldcg(&C[b]), ce = ldcg(&C[b + 1]);
+    *kmin = max((int64_t)cb + 1, (int64_t)c.nmin_count);
+    return *kmin <= ce;
+  };
+  auto nonempty = [&](int32_t b) { return ldcg(&C[b + 1]) > ldcg(&C[b]); };
+  for (int di = 0; di < c.ndelta && ok; di++) {
+    const double dlt = c.delta[di];
+    auto definite = [&](int32_t b) {
+      int64_t k;
+      return bin_k(b, &k) && rad3_for(k, c.mpart, dlt) <= edge3_lo(b, nb, W3);
+    };
+    auto possible = [&](int32_t b) {
+      int64_t k;
+      return bin_k(b, &k) && rad3_for(k, c.mpart, dlt) < edge3_hi(b, nb, W3);
+    };
+    const int32_t b2 = block_first(0, nb - 1, definite, G);
+    if (b2 == INT32_MAX) { ok = false; break; }
+    int32_t last_e = -1, x = 0, ns = 0;
+    for (;;) {
+      const int32_t s0 = block_first(x, b2, possible, G);
+      int32_t e = b2;
+      if (ns < SEGCAP - 1) {
+        const int32_t q = block_first(
+            s0 + 1, b2, [&](int32_t b) { return nonempty(b) && !possible(b); },
+            G);
+        if (q != INT32_MAX) e = q - 1;
+      }
+      // a run close behind the previous segment (few keys between)
+      // extends it instead of opening a segment
+      const bool join =
+          ns > 0 && ldcg(&C[s0]) - ldcg(&C[last_e + 1]) <= GAP_KEYS;
+      const int32_t xs = (!join && s0 > last_e + 1)
+                             ? block_last(last_e + 1, s0 - 1, nonempty, G)
+                             : -1;
+      if (threadIdx.x == 0) {
+        if (join) {
+          G.sg_e[di][ns - 1] = e;
+        } else {
+          G.sg_s[di][ns] = s0;
+          G.sg_e[di][ns] = e;
+          G.sg_x[di][ns] = xs >= 0 ? xs : s0;
+        }
+      }
+      if (!join) ns++;
+      if (e >= b2) break;
+      last_e = e;
+      x = e + 1;
+    }
+    if (threadIdx.x == 0) G.nseg[di] = ns;
+    __syncthreads();
+  }
+  if (!ok) {
+    V.W = 2 * V.W;
+    __syncthreads();
+    return GS_BEGIN;
+  }
+  // ---- disjoint ranges of bins to collect -------------------------------
+  if (threadIdx.x == 0) {
+    int32_t nr = 0;
+    for (int di = 0; di < c.ndelta; di++)
+      for (int j = 0; j < G.nseg[di]; j++) {
+        int32_t a = G.sg_x[di][j], t = G.sg_e[di][j];
+        // absorb every range that overlaps or touches [a, t]
+        int32_t w = 0;
+        for (int r = 0; r < nr; r++) {
+          if (!(t < G.rs[r] - 1 || a > G.re[r] + 1)) {
+            a = min(a, G.rs[r]);
+            t = max(t, G.re[r]);
+          } else {
+            G.rs[w] = G.rs[r];
+            G.re[w] = G.re[r];
+            w++;
+          }
+        }
+        G.rs[w] = a;
+        G.re[w] = t;
+        nr = w + 1;
+      }
+    // ascending order of the ranges (ranks then ascend with them)
+    for (int r = 1; r < nr; r++)
+      for (int q = r; q > 0 && G.rs[q - 1] > G.rs[q]; q--) {
+        const int32_t ts = G.rs[q], te = G.re[q];
+        G.rs[q] = G.rs[q - 1];
+        G.re[q] = G.re[q - 1];
+        G.rs[q - 1] = ts;
+        G.re[q - 1] = te;
+      }
+    G.nr = nr;
+    int64_t oo = 0;
+    for (int r = 0; r < nr; r++) {
+      G.k0[r] = ldcg(&C[G.rs[r]]);
+      G.k1[r] = ldcg(&C[G.re[r] + 1]);
+      G.off[r] = oo;
+      G.fill[r] = 0;
+      oo += pool_slots(G.k1[r] - G.k0[r]);
+    }
+    G.bump = oo;  // a re-run starts the slice afresh
+    G.need = oo;
+  }
+  __syncthreads();
+  if (G.need > pool_cap) return GS_FAILED;
+  // ---- band of walk 2 ----------------------------------------------------
+  const int nr = G.nr;
+  float blo2 = -1.0f, bhi2 = -2.0f;  // nothing when there is no range
+  if (nr > 0) bin_band(G.rs[0], G.re[nr - 1], nb, V.scale, W2, &blo2, &bhi2);
+  V.lo2 = blo2;
+  V.hi2 = bhi2;
+  return GS_COLLECT;
+}
+
+// cross, on the keys of walk 2 in pool: GS_PROFILE, GS_BEGIN (larger
+// reach) or GS_FAILED (G.need slots beyond pool_cap)
+__device__ inline int32_t grow_cross(const SodCfg &c, GrowShared &G,
+                                     const int32_t *hist, uint64_t *pool,
+                                     const float *pos, GrowVars &V,
+                                     HaloResult &res, const double *xcs,
+                                     int64_t pool_cap) {
+  const int32_t *C = hist;
+  const int32_t nb = V.nb;
+  const float scale = V.scale;
+  const float W2 = (float)V.W * (float)V.W;
+  const double Wd = (double)V.W, W3 = Wd * Wd * Wd;
+  const int nr = G.nr;
+  for (int r = 0; r < nr; r++)
+    block_sort_pool(pool + G.off[r], (int32_t)(G.k1[r] - G.k0[r]), G);
+  // key of rank k (1-based) among the collected ranges
+  auto key_rank = [&](int64_t k) {
+    int r = 0;
+    while (r + 1 < nr && k > G.k1[r]) r++;
+    return pool[G.off[r] + (k - G.k0[r] - 1)];
+  };
+  auto nonempty = [&](int32_t b) { return ldcg(&C[b + 1]) > ldcg(&C[b]); };
+  // ---- exact first crossing ------------------------------------------
+  int64_t kst[PSOD_MAXDELTA];
+  uint64_t keyR = 0;
+  bool ok = true;
+  for (int di = 0; di < c.ndelta; di++) {
+    int32_t firstk = INT32_MAX;
+    for (int j = 0; j < G.nseg[di] && firstk == INT32_MAX; j++) {
+      const int64_t klo = max((int64_t)ldcg(&C[G.sg_s[di][j]]) + 1,
+                              (int64_t)c.nmin_count);
+      const int64_t khi = ldcg(&C[G.sg_e[di][j] + 1]);
+      int32_t fk = INT32_MAX;
+      for (int64_t k = klo + threadIdx.x; k <= khi; k += GL_T) {
+        const double rk = (double)sqrtf(key_d2(key_rank(k)));
+        if (od_k(k, c.mpart, rk) <= c.delta[di]) {
+          fk = (int32_t)k;
+          break;
+        }
+      }
+      firstk = block_min_i32(fk, G);
+    }
+    kst[di] = firstk == INT32_MAX ? -1 : firstk;
+    if (kst[di] < 0) { ok = false; break; }
+    const uint64_t k1 = key_rank(kst[di] - 1);
+    const double r1 = ref_dist(pos, key_idx(k1), xcs, c.nx);
+    const double r2 = ref_dist(pos, key_idx(key_rank(kst[di])), xcs, c.nx);
+    finish_delta(c, di, kst[di], r1, r2, res);
+    if (di == 0) keyR = k1;
+  }
+  if (!ok) {
+    V.W = 2 * V.W;
+    __syncthreads();
+    return GS_BEGIN;
+  }
+  const double R0 = res.radius[0];
+  if (c.rmax_frac * R0 > Wd) {
+    const int32_t need_w = (int32_t)ceil(c.rmax_frac * R0) + 1;
+    V.W = max(2 * V.W, need_w);
+    __syncthreads();
+    return GS_BEGIN;
+  }
+  // ---- second-threshold bins (last k above delta2) ----------------------
+  // A bin may hold a rank k <= k0s with od_k > delta2 when its largest
+  // such rank could at the bin's inner edge (possible), and must at the
+  // outer edge (definite).  The answer lies in a possible bin between
+  // the last definite bin (or bin 0) and the last possible bin; those
+  // bins are collected as up to SEGCAP segments found downward, the
+  // lowest one extended to the floor.
+  const int64_t k0s = kst[0];
+  int ns2 = 0;
+  if (c.delta2 > 0.0) {
+    auto kmax2 = [&](int32_t b, int64_t *kmax) {
+      const int32_t cb = ldcg(&C[b]), ce = ldcg(&C[b + 1]);
+      *kmax = min((int64_t)ce, k0s);
+      return *kmax >= max((int64_t)cb + 1, (int64_t)2);
+    };
+    auto possible2 = [&](int32_t b) {
+      int64_t k;
+      return kmax2(b, &k) && rad3_for(k, c.mpart, c.delta2) > edge3_lo(b, nb, W3);
+    };
+    auto definite2 = [&](int32_t b) {
+      int64_t k;
+      return kmax2(b, &k) && rad3_for(k, c.mpart, c.delta2) >= edge3_hi(b, nb, W3);
+    };
+    // ranks above k0s do not count: no bin beyond the one of rank k0s
+    const int32_t bk0 = rbin(key_d2(key_rank(k0s)), scale, nb);
+    const int32_t b3 = block_last(0, bk0, possible2, G);
+    if (b3 >= 0) {
+      const int32_t b4 = block_last(0, b3, definite2, G);
+      const int32_t floor2 = b4 >= 0 ? b4 : 0;
+      int32_t x = b3;
+      while (x >= floor2) {
+        const int32_t e = block_last(floor2, x, possible2, G);
+        if (e < 0) break;
+        int32_t s0 = floor2;
+        if (ns2 < SEGCAP - 1) {
+          const int32_t q = block_last(
+              floor2, e - 1,
+              [&](int32_t b) { return nonempty(b) && !possible2(b); }, G);
+          if (q >= 0) s0 = q + 1;
+        }
+        // join the segment above when few keys lie between
+        const bool join = ns2 > 0 && ldcg(&C[x + 1]) - ldcg(&C[e + 1]) <= GAP_KEYS;
+        if (threadIdx.x == 0) {
+          if (join) {
+            G.rs[R2 + ns2 - 1] = s0;
+          } else {
+            G.rs[R2 + ns2] = s0;
+            G.re[R2 + ns2] = e;
+          }
+        }
+        if (!join) ns2++;
+        if (s0 <= floor2) break;
+        x = s0 - 1;
+      }
+    }
+    if (threadIdx.x == 0) {
+      int64_t o = G.bump;
+      for (int j = 0; j < ns2; j++) {
+        G.k0[R2 + j] = ldcg(&C[G.rs[R2 + j]]);
+        G.k1[R2 + j] = ldcg(&C[G.re[R2 + j] + 1]);
+        G.off[R2 + j] = o;
+        G.fill[R2 + j] = 0;
+        o += pool_slots(G.k1[R2 + j] - G.k0[R2 + j]);
+      }
+      G.bump = o;
+      G.need = o;
+    }
+  }
+  // profile radii
+  for (int j = threadIdx.x; j < c.nbins; j += GL_T) {
+    G.prad[j] = prof_radius(c, R0, j);
+    for (int w = 0; w < GL_T / 64; w++) G.pcnt[w][j] = 0;
+  }
+  __syncthreads();
+  if (G.need > pool_cap) return GS_FAILED;
+  // ---- band of walk 3 ----------------------------------------------------
+  const double rlast = G.prad[c.nbins - 1];
+  // farthest particle walk 3 uses: profile, boundary band, centre of mass
+  // (keys up to keyR) and the second threshold's bins
+  float phi2;
+  {
+    const double rm = fmax(rlast, R0 + c.tol_abs);
+    float h2 = fmaxf((float)(rm * rm * (1.0 + 1e-4)) + 1e-6f, key_d2(keyR));
+    for (int j = 0; j < ns2; j++) {
+      float l2, u2;
+      bin_band(G.rs[R2 + j], G.re[R2 + j], nb, scale, W2, &l2, &u2);
+      h2 = fmaxf(h2, u2);
+    }
+    phi2 = fminf(h2, W2 * (1.0f + 1e-4f) + 1e-3f);
+  }
+  V.k0s = k0s;
+  V.keyR = keyR;
+  V.R0 = R0;
+  V.rlast = rlast;
+  V.ns2 = ns2;
+  V.lo2 = -1.0f;
+  V.hi2 = phi2;
+  return GS_PROFILE;
+}
+
+// finish, on the sums of walk 3 (S, block totals) and its profile counts
+// per bin in G.pcnt[0]: the halo's outputs in V.res, its cumulative
+// profile to out
+__device__ inline void grow_finish(const SodCfg &c, GrowShared &G,
+                                   uint64_t *pool, const float *pos,
+                                   const GrowVars &V, HaloResult &res,
+                                   const double *xcs, const ProfileSums &S,
+                                   psod_halo_out out) {
+  const CellRef &cr = V.cr;
+  const int32_t nin = res.count[0];
+  const double xc[3] = {(double)cr.c[0] + (double)cr.f[0],
+                        (double)cr.c[1] + (double)cr.f[1],
+                        (double)cr.c[2] + (double)cr.f[2]};
+  for (int d = 0; d < 3; d++) {
+    double cp = (xc[d] + S.sp[d] / (double)nin) / (double)c.nx;
+    cp -= floor(cp);
+    res.com_pos[d] = cp;
+    res.com_vel[d] = S.sv[d] / (double)nin;
+  }
+  // cumulative profile
+  if (threadIdx.x == 0) {
+    int32_t s = 0;
+    for (int j = 0; j < c.nbins; j++) {
+      s += G.pcnt[0][j];
+      out.profile[V.h * c.nbins + j] = s;
+    }
+  }
+  // ---- delta2 --------------------------------------------------------
+  if (c.delta2 > 0.0) {
+    int64_t best = 0;
+    uint64_t bkey = 0;
+    if (V.ns2 > 0) {
+      __threadfence_block();
+      __syncthreads();
+      // segments from the top down: the first one holding a rank above
+      // delta2 holds the last such rank
+      for (int j = 0; j < V.ns2 && best == 0; j++) {
+        const int32_t n2 = (int32_t)(G.k1[R2 + j] - G.k0[R2 + j]);
+        uint64_t *p2 = pool + G.off[R2 + j];
+        block_sort_pool(p2, n2, G);
+        const int64_t kb = G.k0[R2 + j];
+        int32_t bestk = -1;
+        for (int32_t t = threadIdx.x; t < n2; t += GL_T) {
+          const int64_t k = kb + t + 1;
+          if (k < 2 || k > V.k0s) continue;
+          const double rk = (double)sqrtf(key_d2(p2[t]));
+          if (od_k(k, c.mpart, rk) > c.delta2) bestk = max(bestk, (int32_t)k);
+        }
+        bestk = block_max_i32(bestk, G);
+        if (bestk > 0) {
+          best = bestk;
+          bkey = p2[bestk - kb - 1];
+        }
+      }
+    }
+    res.count2 = (int32_t)best;
+    if (best > 0) {
+      const double rb = ref_dist(pos, key_idx(bkey), xcs, c.nx);
+      res.overdens2 = rb > 1e-30 ? od_k(best, c.mpart, rb) : 0.0;
+    }
+  }
+  res.w = V.W;
+  res.ngather = (int32_t)V.ncube;
+  res.nboundary = (int32_t)S.nbd;
+  bool acc = (double)res.count[0] * c.mpart >= c.mmin;
+  for (int d = 0; d < 3; d++) acc = acc && xc[d] > 0.0 && xc[d] <= c.nx;
+  res.status = PSOD_H_CLOSED | PSOD_H_LARGE | (acc ? PSOD_H_ACCEPTED : 0);
+}
+
+// Stores a halo's outputs (thread 0); a failed one (its bands beyond the
+// slice) goes to the retry list with the slice size it needs.
+__device__ inline void grow_close(const SodCfg &c, int32_t *counters,
+                                  int32_t *retry,
+                                  unsigned long long *needmax, int64_t need,
+                                  const GrowVars &V, HaloResult &res,
+                                  bool failed, psod_halo_out out) {
+  if (failed) res.status = PSOD_H_LARGE;
+  if (threadIdx.x == 0) {
+    if (failed) {
+      const int32_t k = atomicAdd(&counters[4], 1);
+      if (retry) retry[k] = V.e;
+      if (needmax) atomicMax(needmax, (unsigned long long)need);
+    }
+    if (res.status & PSOD_H_UNCLOSED) atomicAdd(&counters[5], 1);
+    store_result(out, V.h, c.nbins, res);
   }
 }
 
@@ -1808,11 +2508,16 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
              int64_t pool_cap, const int32_t *__restrict__ elist,
              int32_t *__restrict__ retry,
              unsigned long long *__restrict__ needmax,
-             const int32_t *__restrict__ abort_flag, psod_halo_out out) {
+             const int32_t *__restrict__ abort_flag,
+             int32_t *__restrict__ giants, int32_t giant_cap,
+             psod_halo_out out) {
   // Work items are positions e of the large list (or elist[item] when a
   // list of positions is given).  A halo whose bands do not fit the slice
   // is appended to `retry` (when given) with the slice size it needs
-  // folded into *needmax, for a second launch with larger slices.
+  // folded into *needmax, for a second launch with larger slices.  With
+  // `giants`, a halo whose cube reaches c.giant_min particles is appended
+  // there as (e, W) instead (counted in counters[6], at most giant_cap
+  // entries) and left to the staged launches.
   __shared__ GrowShared G;
   __shared__ int32_t lh[HIST_LDS];  // inner bins; staging of the scan
   // a captured pass whose centre search overflowed: the host runs the pass
@@ -1829,469 +2534,514 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
     const int32_t item = G.item;
     __syncthreads();
     if (item >= nitem) break;
-    const int32_t e = elist ? elist[item] : item;
-    const int64_t h = large[e];
-    const HaloPlan pl = plan[h];
-    const int32_t centre = pl.centre;
-    const CellRef cr = ref_of_particle(pos, centre, c.nx);
+    GrowVars V;
     HaloResult res;
-    zero_result(res);
-    res.centre = centre;
-    res.centre_id = ids ? ids[centre] : (int64_t)centre;
-    int32_t W = pl.w0 < 2 ? 2 : pl.w0;
-    bool failed = false;
-    if (threadIdx.x == 0) G.bump = 0;
     double xcs[3];
-    centre_scaled(pos, centre, c.nx, xcs);
-    for (;;) {
-      if (2 * W + 1 > c.nx) {
-        res.status = PSOD_H_UNCLOSED | PSOD_H_LARGE;
-        res.w = W;
-        break;
+    grow_start(c, pos, ids, plan, large, elist ? elist[item] : item, V, res,
+               xcs);
+    if (threadIdx.x == 0) G.bump = 0;
+    int32_t step = GS_BEGIN;
+    bool deferred = false;
+    while (step == GS_BEGIN) {
+      step = grow_begin(c, G, cell_start, V, res);
+      if (step != GS_HIST) break;
+      if (giants && V.ncube >= c.giant_min) {
+        if (threadIdx.x == 0) {
+          const int32_t k = atomicAdd(&counters[6], 1);
+          G.item = k < giant_cap ? 1 : 0;
+          if (k < giant_cap) {
+            giants[2 * k] = V.e;
+            giants[2 * k + 1] = V.W;
+          }
+        }
+        __syncthreads();
+        deferred = G.item != 0;
+        __syncthreads();
+        if (deferred) break;
       }
-      const float W2 = (float)W * (float)W;
-      // bins uniform in r^3, about 4 particles of the cube per bin
-      const int64_t ncube = cube_count(c, G, cell_start, cr, W);
-      int32_t nb = HIST_MIN;
-      while ((int64_t)nb * 4 < ncube && nb < HIST_MAX) nb <<= 1;
-      const float scale = (float)nb / ((float)W * (float)W * (float)W);
-      const int32_t nl = min(nb, HIST_LDS);
+      const int32_t nl = min(V.nb, HIST_LDS);
       for (int32_t b = threadIdx.x; b < nl; b += GL_T) lh[b] = 0;
-      for (int32_t b = nl + threadIdx.x; b <= nb; b += GL_T) hist[b] = 0;
+      for (int32_t b = nl + threadIdx.x; b <= V.nb; b += GL_T) hist[b] = 0;
       __threadfence_block();
       __syncthreads();
-      // ---- pass 1: histogram -------------------------------------------
-      // every particle of the cube is gathered; those of d2 < W2 are binned
-      const int64_t ng = ncube;
-      cube_walk(c, G, cell_start, rec, cr, W, -1.0f,
-                W2 * (1.0f + 1e-4f) + 1e-3f,
-                [&](float d2, int32_t, const float4 &, int, int, int,
-                    const int32_t *) {
-                  if (d2 < W2) {
-                    const int32_t b = rbin(d2, scale, nb);
-                    if (b < HIST_LDS) atomicAdd(&lh[b], 1);
-                    else atomicAdd(&hist[b], 1);
-                  }
-                });
+      walk_hist(c, G, lh, hist, cell_start, rec, V);
       for (int32_t b = threadIdx.x; b < nl; b += GL_T) hist[b] = lh[b];
       __threadfence();
       __syncthreads();
-      // exclusive scan in place: hist[b] = #(bin < b), hist[nb] = total.
-      // Rounds of SCAN_ROUND bins, read coalesced into LDS; thread t scans
-      // the SCAN_ROUND / GL_T consecutive bins from t * SCAN_ROUND / GL_T.
-      {
-        constexpr int PER = SCAN_ROUND / GL_T;
-        const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-        int32_t carry = 0;
-        for (int32_t base = 0; base < nb; base += SCAN_ROUND) {
-          for (int i = threadIdx.x; i < SCAN_ROUND; i += GL_T)
-            lh[i] = ldcg(&hist[base + i]);
-          __syncthreads();
-          int32_t v[PER], sum = 0;
-#pragma unroll
-          for (int i = 0; i < PER; i++) {
-            v[i] = lh[threadIdx.x * PER + i];
-            sum += v[i];
-          }
-          int32_t tot;
-          const int32_t ex = wave_excl(sum, &tot);
-          if (l == 0) G.red_i[w] = tot;
-          __syncthreads();
-          int32_t run = carry + ex, all = 0;
-          for (int i = 0; i < GL_T / 64; i++) {
-            if (i < w) run += G.red_i[i];
-            all += G.red_i[i];
-          }
-#pragma unroll
-          for (int i = 0; i < PER; i++) {
-            lh[threadIdx.x * PER + i] = run;
-            run += v[i];
-          }
-          __syncthreads();
-          for (int i = threadIdx.x; i < SCAN_ROUND; i += GL_T)
-            hist[base + i] = lh[i];
-          carry += all;
-          __syncthreads();
-        }
-        if (threadIdx.x == 0) hist[nb] = carry;
-        __threadfence();
-        __syncthreads();
-      }
-      const int32_t *C = hist;
-      const double Wd = (double)W, W3 = Wd * Wd * Wd;
-      // ---- bins that may hold each threshold's first crossing --------------
-      // A bin is possible for delta when the smallest admissible rank k in it
-      // could have od_k <= delta at the bin's outer edge, and definite when
-      // it must at the inner edge; the crossing lies in a possible bin no
-      // later than the first definite one.  The possible bins are collected
-      // as up to SEGCAP segments, each extended down to the previous
-      // non-empty bin, which holds rank k-1 when k opens the segment.
-      bool ok = true;
-      auto bin_k = [&](int32_t b, int64_t *kmin) {
-        const int32_t cb = ldcg(&C[b]), ce = ldcg(&C[b + 1]);
-        *kmin = max((int64_t)cb + 1, (int64_t)c.nmin_count);
-        return *kmin <= ce;
-      };
-      auto nonempty = [&](int32_t b) { return ldcg(&C[b + 1]) > ldcg(&C[b]); };
-      for (int di = 0; di < c.ndelta && ok; di++) {
-        const double dlt = c.delta[di];
-        auto definite = [&](int32_t b) {
-          int64_t k;
-          return bin_k(b, &k) && rad3_for(k, c.mpart, dlt) <= edge3_lo(b, nb, W3);
-        };
-        auto possible = [&](int32_t b) {
-          int64_t k;
-          return bin_k(b, &k) && rad3_for(k, c.mpart, dlt) < edge3_hi(b, nb, W3);
-        };
-        const int32_t b2 = block_first(0, nb - 1, definite, G);
-        if (b2 == INT32_MAX) { ok = false; break; }
-        int32_t last_e = -1, x = 0, ns = 0;
-        for (;;) {
-          const int32_t s0 = block_first(x, b2, possible, G);
-          int32_t e = b2;
-          if (ns < SEGCAP - 1) {
-            const int32_t q = block_first(
-                s0 + 1, b2, [&](int32_t b) { return nonempty(b) && !possible(b); },
-                G);
-            if (q != INT32_MAX) e = q - 1;
-          }
-          // a run close behind the previous segment (few keys between)
-          // extends it instead of opening a segment
-          const bool join =
-              ns > 0 && ldcg(&C[s0]) - ldcg(&C[last_e + 1]) <= GAP_KEYS;
-          const int32_t xs = (!join && s0 > last_e + 1)
-                                 ? block_last(last_e + 1, s0 - 1, nonempty, G)
-                                 : -1;
-          if (threadIdx.x == 0) {
-            if (join) {
-              G.sg_e[di][ns - 1] = e;
-            } else {
-              G.sg_s[di][ns] = s0;
-              G.sg_e[di][ns] = e;
-              G.sg_x[di][ns] = xs >= 0 ? xs : s0;
-            }
-          }
-          if (!join) ns++;
-          if (e >= b2) break;
-          last_e = e;
-          x = e + 1;
-        }
-        if (threadIdx.x == 0) G.nseg[di] = ns;
-        __syncthreads();
-      }
-      if (!ok) {
-        W = 2 * W;
-        __syncthreads();
-        continue;
-      }
-      // ---- disjoint ranges of bins to collect -------------------------------
-      if (threadIdx.x == 0) {
-        int32_t nr = 0;
-        for (int di = 0; di < c.ndelta; di++)
-          for (int j = 0; j < G.nseg[di]; j++) {
-            int32_t a = G.sg_x[di][j], t = G.sg_e[di][j];
-            // absorb every range that overlaps or touches [a, t]
-            int32_t w = 0;
-            for (int r = 0; r < nr; r++) {
-              if (!(t < G.rs[r] - 1 || a > G.re[r] + 1)) {
-                a = min(a, G.rs[r]);
-                t = max(t, G.re[r]);
-              } else {
-                G.rs[w] = G.rs[r];
-                G.re[w] = G.re[r];
-                w++;
-              }
-            }
-            G.rs[w] = a;
-            G.re[w] = t;
-            nr = w + 1;
-          }
-        // ascending order of the ranges (ranks then ascend with them)
-        for (int r = 1; r < nr; r++)
-          for (int q = r; q > 0 && G.rs[q - 1] > G.rs[q]; q--) {
-            const int32_t ts = G.rs[q], te = G.re[q];
-            G.rs[q] = G.rs[q - 1];
-            G.re[q] = G.re[q - 1];
-            G.rs[q - 1] = ts;
-            G.re[q - 1] = te;
-          }
-        G.nr = nr;
-        int64_t oo = 0;
-        for (int r = 0; r < nr; r++) {
-          G.k0[r] = ldcg(&C[G.rs[r]]);
-          G.k1[r] = ldcg(&C[G.re[r] + 1]);
-          G.off[r] = oo;
-          G.fill[r] = 0;
-          oo += pool_slots(G.k1[r] - G.k0[r]);
-        }
-        G.bump = oo;  // a re-run starts the slice afresh
-        G.need = oo;
-      }
+      step = grow_bins(c, G, lh, hist, V, pool_cap);
+      if (step != GS_COLLECT) continue;
+      walk_collect(c, G, cell_start, rec, V, G.nr, G.fill, pool);
+      __threadfence();
       __syncthreads();
-      const int nr = G.nr;
-      if (G.need > pool_cap) { failed = true; break; }
-      // ---- pass 2: collect the ranges ---------------------------------------
-      float blo2 = -1.0f, bhi2 = -2.0f;  // nothing when there is no range
-      if (nr > 0) bin_band(G.rs[0], G.re[nr - 1], nb, scale, W2, &blo2, &bhi2);
-      cube_walk(c, G, cell_start, rec, cr, W, blo2, bhi2,
-                [&](float d2, int32_t idx, const float4 &, int, int, int,
-                    const int32_t *) {
-                  if (!(d2 < W2)) return;
-                  const int32_t b = rbin(d2, scale, nb);
-                  for (int r = 0; r < nr; r++)
-                    if (b >= G.rs[r] && b <= G.re[r]) {
-                      const int32_t s = atomicAdd(&G.fill[r], 1);
-                      pool[G.off[r] + s] = d2key(d2, idx);
-                    }
-                });
-      __threadfence_block();
-      __syncthreads();
-      for (int r = 0; r < nr; r++)
-        block_sort_pool(pool + G.off[r], (int32_t)(G.k1[r] - G.k0[r]), G);
-      // key of rank k (1-based) among the collected ranges
-      auto key_rank = [&](int64_t k) {
-        int r = 0;
-        while (r + 1 < nr && k > G.k1[r]) r++;
-        return pool[G.off[r] + (k - G.k0[r] - 1)];
-      };
-      // ---- exact first crossing ------------------------------------------
-      int64_t kst[PSOD_MAXDELTA];
-      uint64_t keyR = 0;
-      for (int di = 0; di < c.ndelta; di++) {
-        int32_t firstk = INT32_MAX;
-        for (int j = 0; j < G.nseg[di] && firstk == INT32_MAX; j++) {
-          const int64_t klo = max((int64_t)ldcg(&C[G.sg_s[di][j]]) + 1,
-                                  (int64_t)c.nmin_count);
-          const int64_t khi = ldcg(&C[G.sg_e[di][j] + 1]);
-          int32_t fk = INT32_MAX;
-          for (int64_t k = klo + threadIdx.x; k <= khi; k += GL_T) {
-            const double rk = (double)sqrtf(key_d2(key_rank(k)));
-            if (od_k(k, c.mpart, rk) <= c.delta[di]) {
-              fk = (int32_t)k;
-              break;
-            }
-          }
-          firstk = block_min_i32(fk, G);
-        }
-        kst[di] = firstk == INT32_MAX ? -1 : firstk;
-        if (kst[di] < 0) { ok = false; break; }
-        const uint64_t k1 = key_rank(kst[di] - 1);
-        const double r1 = ref_dist(pos, key_idx(k1), xcs, c.nx);
-        const double r2 = ref_dist(pos, key_idx(key_rank(kst[di])), xcs, c.nx);
-        finish_delta(c, di, kst[di], r1, r2, res);
-        if (di == 0) keyR = k1;
-      }
-      if (!ok) {
-        W = 2 * W;
-        __syncthreads();
-        continue;
-      }
-      const double R0 = res.radius[0];
-      if (c.rmax_frac * R0 > Wd) {
-        const int32_t need_w = (int32_t)ceil(c.rmax_frac * R0) + 1;
-        W = max(2 * W, need_w);
-        __syncthreads();
-        continue;
-      }
-      // ---- second-threshold bins (last k above delta2) ----------------------
-      // A bin may hold a rank k <= k0s with od_k > delta2 when its largest
-      // such rank could at the bin's inner edge (possible), and must at the
-      // outer edge (definite).  The answer lies in a possible bin between
-      // the last definite bin (or bin 0) and the last possible bin; those
-      // bins are collected as up to SEGCAP segments found downward, the
-      // lowest one extended to the floor.
-      const int64_t k0s = kst[0];
-      int ns2 = 0;
-      if (c.delta2 > 0.0) {
-        auto kmax2 = [&](int32_t b, int64_t *kmax) {
-          const int32_t cb = ldcg(&C[b]), ce = ldcg(&C[b + 1]);
-          *kmax = min((int64_t)ce, k0s);
-          return *kmax >= max((int64_t)cb + 1, (int64_t)2);
-        };
-        auto possible2 = [&](int32_t b) {
-          int64_t k;
-          return kmax2(b, &k) && rad3_for(k, c.mpart, c.delta2) > edge3_lo(b, nb, W3);
-        };
-        auto definite2 = [&](int32_t b) {
-          int64_t k;
-          return kmax2(b, &k) && rad3_for(k, c.mpart, c.delta2) >= edge3_hi(b, nb, W3);
-        };
-        // ranks above k0s do not count: no bin beyond the one of rank k0s
-        const int32_t bk0 = rbin(key_d2(key_rank(k0s)), scale, nb);
-        const int32_t b3 = block_last(0, bk0, possible2, G);
-        if (b3 >= 0) {
-          const int32_t b4 = block_last(0, b3, definite2, G);
-          const int32_t floor2 = b4 >= 0 ? b4 : 0;
-          int32_t x = b3;
-          while (x >= floor2) {
-            const int32_t e = block_last(floor2, x, possible2, G);
-            if (e < 0) break;
-            int32_t s0 = floor2;
-            if (ns2 < SEGCAP - 1) {
-              const int32_t q = block_last(
-                  floor2, e - 1,
-                  [&](int32_t b) { return nonempty(b) && !possible2(b); }, G);
-              if (q >= 0) s0 = q + 1;
-            }
-            // join the segment above when few keys lie between
-            const bool join = ns2 > 0 && ldcg(&C[x + 1]) - ldcg(&C[e + 1]) <= GAP_KEYS;
-            if (threadIdx.x == 0) {
-              if (join) {
-                G.rs[R2 + ns2 - 1] = s0;
-              } else {
-                G.rs[R2 + ns2] = s0;
-                G.re[R2 + ns2] = e;
-              }
-            }
-            if (!join) ns2++;
-            if (s0 <= floor2) break;
-            x = s0 - 1;
-          }
-        }
-        if (threadIdx.x == 0) {
-          int64_t o = G.bump;
-          for (int j = 0; j < ns2; j++) {
-            G.k0[R2 + j] = ldcg(&C[G.rs[R2 + j]]);
-            G.k1[R2 + j] = ldcg(&C[G.re[R2 + j] + 1]);
-            G.off[R2 + j] = o;
-            G.fill[R2 + j] = 0;
-            o += pool_slots(G.k1[R2 + j] - G.k0[R2 + j]);
-          }
-          G.bump = o;
-          G.need = o;
-        }
-      }
-      // profile radii
-      for (int j = threadIdx.x; j < c.nbins; j += GL_T) {
-        G.prad[j] = prof_radius(c, R0, j);
-        for (int w = 0; w < GL_T / 64; w++) G.pcnt[w][j] = 0;
-      }
-      __syncthreads();
-      if (G.need > pool_cap) { failed = true; break; }
-      // ---- pass 3: profile, centre of mass, boundary, band of delta2 -------
-      const double rlast = G.prad[c.nbins - 1];
-      double sp[3] = {0, 0, 0}, sv[3] = {0, 0, 0};
-      int64_t nbd = 0;
+      step = grow_cross(c, G, hist, pool, pos, V, res, xcs, pool_cap);
+      if (step != GS_PROFILE) continue;
       const int flagv = (c.flags & PSOD_F_VELOCITY) && vel;
-      // farthest particle this pass uses: profile, boundary band, centre of
-      // mass (keys up to keyR) and the second threshold's bins
-      float phi2;
-      {
-        const double rm = fmax(rlast, R0 + c.tol_abs);
-        float h2 = fmaxf((float)(rm * rm * (1.0 + 1e-4)) + 1e-6f, key_d2(keyR));
-        for (int j = 0; j < ns2; j++) {
-          float l2, u2;
-          bin_band(G.rs[R2 + j], G.re[R2 + j], nb, scale, W2, &l2, &u2);
-          h2 = fmaxf(h2, u2);
-        }
-        phi2 = fminf(h2, W2 * (1.0f + 1e-4f) + 1e-3f);
-      }
-      cube_walk(c, G, cell_start, rec, cr, W, -1.0f, phi2,
-                [&](float d2, int32_t idx, const float4 &r, int di, int dj,
-                    int dk, const int32_t *lc) {
-                  if (!(d2 < W2)) return;
-                  const double rr = (double)sqrtf(d2);
-                  if (rr < rlast) {
-                    int lo = 0, hi = c.nbins - 1;  // first j with rr < prad[j]
-                    while (lo < hi) {
-                      const int mid = (lo + hi) >> 1;
-                      if (rr < G.prad[mid]) hi = mid; else lo = mid + 1;
-                    }
-                    atomicAdd(&G.pcnt[threadIdx.x >> 6][lo], 1);
-                  }
-                  if (fabs(rr - R0) <= c.tol_abs) nbd++;
-                  if (d2key(d2, idx) <= keyR) {
-                    sp[0] += rel1d(r.x, lc[0], di, (double)cr.f[0]);
-                    sp[1] += rel1d(r.y, lc[1], dj, (double)cr.f[1]);
-                    sp[2] += rel1d(r.z, lc[2], dk, (double)cr.f[2]);
-                    if (flagv)
-                      for (int d = 0; d < 3; d++)
-                        sv[d] += (double)vel[3 * (int64_t)idx + d];
-                  }
-                  if (ns2 > 0) {
-                    const int32_t b = rbin(d2, scale, nb);
-                    for (int j = 0; j < ns2; j++)
-                      if (b >= G.rs[R2 + j] && b <= G.re[R2 + j]) {
-                        const int32_t s = atomicAdd(&G.fill[R2 + j], 1);
-                        pool[G.off[R2 + j] + s] = d2key(d2, idx);
-                      }
-                  }
-                });
-      nbd = block_sum_i64(nbd, G);
-      const int32_t nin = res.count[0];
-      const double xc[3] = {(double)cr.c[0] + (double)cr.f[0],
-                            (double)cr.c[1] + (double)cr.f[1],
-                            (double)cr.c[2] + (double)cr.f[2]};
+      ProfileSums S = {{0, 0, 0}, {0, 0, 0}, 0};
+      walk_profile(c, G, cell_start, rec, vel, V, flagv, G.fill, pool, S);
+      S.nbd = block_sum_i64(S.nbd, G);
       for (int d = 0; d < 3; d++) {
-        const double s = block_sum_f64(sp[d], G);
-        const double v = block_sum_f64(sv[d], G);
-        double cp = (xc[d] + s / (double)nin) / (double)c.nx;
-        cp -= floor(cp);
-        res.com_pos[d] = cp;
-        res.com_vel[d] = v / (double)nin;
+        S.sp[d] = block_sum_f64(S.sp[d], G);
+        S.sv[d] = block_sum_f64(S.sv[d], G);
       }
-      // cumulative profile
-      if (threadIdx.x == 0) {
+      // per-bin counts of all waves in G.pcnt[0]
+      for (int j = threadIdx.x; j < c.nbins; j += GL_T) {
         int32_t s = 0;
-        for (int j = 0; j < c.nbins; j++) {
-          for (int w = 0; w < GL_T / 64; w++) s += G.pcnt[w][j];
-          out.profile[h * c.nbins + j] = s;
-        }
+        for (int w = 0; w < GL_T / 64; w++) s += G.pcnt[w][j];
+        G.pcnt[0][j] = s;
       }
-      // ---- delta2 --------------------------------------------------------
-      if (c.delta2 > 0.0) {
-        int64_t best = 0;
-        uint64_t bkey = 0;
-        if (ns2 > 0) {
-          __threadfence_block();
-          __syncthreads();
-          // segments from the top down: the first one holding a rank above
-          // delta2 holds the last such rank
-          for (int j = 0; j < ns2 && best == 0; j++) {
-            const int32_t n2 = (int32_t)(G.k1[R2 + j] - G.k0[R2 + j]);
-            uint64_t *p2 = pool + G.off[R2 + j];
-            block_sort_pool(p2, n2, G);
-            const int64_t kb = G.k0[R2 + j];
-            int32_t bestk = -1;
-            for (int32_t t = threadIdx.x; t < n2; t += GL_T) {
-              const int64_t k = kb + t + 1;
-              if (k < 2 || k > k0s) continue;
-              const double rk = (double)sqrtf(key_d2(p2[t]));
-              if (od_k(k, c.mpart, rk) > c.delta2) bestk = max(bestk, (int32_t)k);
-            }
-            bestk = block_max_i32(bestk, G);
-            if (bestk > 0) {
-              best = bestk;
-              bkey = p2[bestk - kb - 1];
-            }
-          }
-        }
-        res.count2 = (int32_t)best;
-        if (best > 0) {
-          const double rb = ref_dist(pos, key_idx(bkey), xcs, c.nx);
-          res.overdens2 = rb > 1e-30 ? od_k(best, c.mpart, rb) : 0.0;
-        }
-      }
-      res.w = W;
-      res.ngather = (int32_t)ng;
-      res.nboundary = (int32_t)nbd;
-      bool acc = (double)res.count[0] * c.mpart >= c.mmin;
-      for (int d = 0; d < 3; d++) acc = acc && xc[d] > 0.0 && xc[d] <= c.nx;
-      res.status = PSOD_H_CLOSED | PSOD_H_LARGE | (acc ? PSOD_H_ACCEPTED : 0);
+      __syncthreads();
+      grow_finish(c, G, pool, pos, V, res, xcs, S, out);
+      step = GS_DONE;
+    }
+    if (!deferred)
+      grow_close(c, counters, retry, needmax, G.need, V, res,
+                 step == GS_FAILED, out);
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Staged growth of giant halos.  GiantState g holds the state of giant halo
+// g between launches: its GrowVars, its next phase, and the fields of
+// GrowShared that carry from one phase to the next (ranges, segments, slot
+// counters, profile radii), plus the sums the walks add up over
+// workgroups.  Giant g has its own histogram slice (HIST_SLICE) and
+// collection slice (pool_cap keys).  Every phase kernel skips the giants
+// that are not in its phase, so the host runs cycles of begin, count walk,
+// size, walk 1, bins, walk 2, cross, walk 3, finish until every giant is
+// done or failed; a giant whose crossing lies beyond its cube goes round
+// again with a larger reach.  No launch waits on another workgroup: the
+// phases meet only at launch boundaries.
+// ---------------------------------------------------------------------------
+constexpr int32_t GIANT_CAP = 64;  // giant halos staged per pass
+
+struct GiantState {
+  GrowVars V;
+  HaloResult res;
+  double xcs[3];
+  int32_t step;  // next phase
+  int32_t nch;   // chunks of the phase's walk
+  int32_t cseg;  // segments of the cube
+  int64_t segoff;  // first of its per-segment counts and their prefix sums
+  unsigned long long ncube;  // particles of the cube (the count walk)
+  int64_t need, bump;
+  int32_t nr;
+  int32_t fill[MAXRANGE], rs[MAXRANGE], re[MAXRANGE];
+  int64_t off[MAXRANGE], k0[MAXRANGE], k1[MAXRANGE];
+  int32_t sg_s[PSOD_MAXDELTA][SEGCAP], sg_e[PSOD_MAXDELTA][SEGCAP],
+      sg_x[PSOD_MAXDELTA][SEGCAP], nseg[PSOD_MAXDELTA];
+  double prad[257];
+  // sums of walk 3
+  int32_t pcnt[257];
+  double sp[3], sv[3];
+  unsigned long long nbd;
+};
+
+// GrowShared fields of giant T to LDS (all threads; ends with a barrier)
+__device__ inline void giant_load(GrowShared &G, const GiantState &T) {
+  for (int i = threadIdx.x; i < MAXRANGE; i += GL_T) {
+    G.fill[i] = T.fill[i];
+    G.rs[i] = T.rs[i];
+    G.re[i] = T.re[i];
+    G.off[i] = T.off[i];
+    G.k0[i] = T.k0[i];
+    G.k1[i] = T.k1[i];
+  }
+  for (int i = threadIdx.x; i < PSOD_MAXDELTA * SEGCAP; i += GL_T) {
+    const int di = i / SEGCAP, j = i % SEGCAP;
+    G.sg_s[di][j] = T.sg_s[di][j];
+    G.sg_e[di][j] = T.sg_e[di][j];
+    G.sg_x[di][j] = T.sg_x[di][j];
+  }
+  for (int i = threadIdx.x; i < PSOD_MAXDELTA; i += GL_T) G.nseg[i] = T.nseg[i];
+  for (int j = threadIdx.x; j < 257; j += GL_T) G.prad[j] = T.prad[j];
+  if (threadIdx.x == 0) {
+    G.nr = T.nr;
+    G.bump = T.bump;
+    G.need = T.need;
+  }
+  __syncthreads();
+}
+
+// the same fields back to giant T (all threads, after a barrier), with
+// V (thread 0) and the next phase
+__device__ inline void giant_save(const GrowShared &G, GiantState &T,
+                                  const GrowVars &V, int32_t step,
+                                  int32_t nch) {
+  for (int i = threadIdx.x; i < MAXRANGE; i += GL_T) {
+    T.fill[i] = G.fill[i];
+    T.rs[i] = G.rs[i];
+    T.re[i] = G.re[i];
+    T.off[i] = G.off[i];
+    T.k0[i] = G.k0[i];
+    T.k1[i] = G.k1[i];
+  }
+  for (int i = threadIdx.x; i < PSOD_MAXDELTA * SEGCAP; i += GL_T) {
+    const int di = i / SEGCAP, j = i % SEGCAP;
+    T.sg_s[di][j] = G.sg_s[di][j];
+    T.sg_e[di][j] = G.sg_e[di][j];
+    T.sg_x[di][j] = G.sg_x[di][j];
+  }
+  for (int i = threadIdx.x; i < PSOD_MAXDELTA; i += GL_T) T.nseg[i] = G.nseg[i];
+  for (int j = threadIdx.x; j < 257; j += GL_T) T.prad[j] = G.prad[j];
+  if (threadIdx.x == 0) {
+    T.nr = G.nr;
+    T.bump = G.bump;
+    T.need = G.need;
+    T.V = V;
+    T.step = step;
+    T.nch = nch;
+  }
+}
+
+// giant g from the list written by k_grow_large (e, W); clears the walks'
+// work counters wq[0..3]
+__global__ void k_giant_init(SodCfg c, const float *__restrict__ pos,
+                             const int64_t *__restrict__ ids,
+                             const HaloPlan *__restrict__ plan,
+                             const int32_t *__restrict__ large,
+                             const int32_t *__restrict__ giants,
+                             GiantState *__restrict__ S,
+                             int32_t *__restrict__ wq) {
+  const int32_t g = blockIdx.x;
+  if (threadIdx.x == 0) {
+    GrowVars V;
+    HaloResult res;
+    double xcs[3];
+    grow_start(c, pos, ids, plan, large, giants[2 * g], V, res, xcs);
+    V.W = giants[2 * g + 1];
+    GiantState &T = S[g];
+    T.V = V;
+    T.res = res;
+    for (int d = 0; d < 3; d++) T.xcs[d] = xcs[d];
+    T.step = GS_BEGIN;
+    T.nch = 0;
+    T.need = 0;
+    T.bump = 0;
+    T.nr = 0;
+  }
+  if (g == 0 && threadIdx.x < 4) wq[threadIdx.x] = 0;
+}
+
+// begin for the giants in GS_BEGIN: close an unclosed one, else set up the
+// count walk; clears the counter of its launch
+__global__ void __launch_bounds__(GL_T)
+k_giant_begin(SodCfg c, GiantState *__restrict__ S,
+              int32_t *__restrict__ counters, int32_t *__restrict__ retry,
+              unsigned long long *__restrict__ needmax, psod_halo_out out,
+              int32_t *__restrict__ wq) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) wq[0] = 0;
+  GiantState &T = S[blockIdx.x];
+  const int32_t step0 = T.step;
+  __syncthreads();
+  if (step0 != GS_BEGIN) return;
+  const GrowVars V = T.V;
+  HaloResult res = T.res;
+  if (!grow_reach(c, V, res)) {
+    grow_close(c, counters, retry, needmax, 0, V, res, false, out);
+    if (threadIdx.x == 0) T.step = GS_DONE;
+    return;
+  }
+  if (threadIdx.x == 0) {
+    const int64_t nseg = cube_segs(c, V.cr, V.W).nseg;
+    T.ncube = 0;
+    T.cseg = (int32_t)nseg;
+    T.step = GS_COUNT;
+    T.nch = (int32_t)chunks_of(nseg);
+  }
+}
+
+// Layout of the per-segment counts of the giants in GS_COUNT, one after
+// the other: their offsets to the states, their segment counts to
+// lay[0..ngiant) (0 for the others) and the total to lay[ngiant].
+__global__ void k_giant_layout(GiantState *__restrict__ S, int32_t ngiant,
+                               int64_t *__restrict__ lay) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  int64_t off = 0;
+  for (int32_t g = 0; g < ngiant; g++) {
+    const int64_t n = S[g].step == GS_COUNT ? S[g].cseg : 0;
+    S[g].segoff = off;
+    lay[g] = n;
+    off += n;
+  }
+  lay[ngiant] = off;
+}
+
+// chunks of CH_PART particles of a cube of ncube particles
+__device__ inline int32_t part_chunks(int64_t ncube) {
+  return (int32_t)max((int64_t)1, (ncube + CH_PART - 1) / CH_PART);
+}
+
+// First s in [0, n) with pre[s] >= x, or n, for a non-decreasing pre: wave
+// 0 of the workgroup, 64 probes per round, the answer in all its lanes.
+__device__ inline int32_t wave_lower_bound(const int32_t *pre, int32_t n,
+                                           int64_t x) {
+  const int l = threadIdx.x & 63;
+  int32_t lo = 0, hi = n;  // the answer lies in [lo, hi]
+  while (hi > lo) {
+    const int32_t step = (hi - lo + 63) / 64;
+    // lane l probes the last index of its part [lo + l step, lo + (l+1) step)
+    const int64_t p = (int64_t)lo + (int64_t)(l + 1) * step - 1;
+    const bool t = p >= hi || (int64_t)pre[p] >= x;
+    const uint64_t m = __ballot(t);
+    if (m == 0) {  // every probe below hi is short of x
+      lo = hi;
       break;
     }
-    if (failed) res.status = PSOD_H_LARGE;  // band beyond the slice
+    const int f = __ffsll((unsigned long long)m) - 1;
+    const int32_t nhi = (int32_t)min((int64_t)hi, (int64_t)lo + (int64_t)(f + 1) * step - 1);
+    lo = lo + f * step;
+    hi = nhi;
+  }
+  return lo;
+}
+
+// size for the giants in GS_COUNT: the histogram of the counted cube,
+// cleared for walk 1; clears the counter of its launch
+__global__ void __launch_bounds__(GL_T)
+k_giant_size(SodCfg c, GiantState *__restrict__ S,
+             int32_t *__restrict__ hist_all, int32_t *__restrict__ wq) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) wq[1] = 0;
+  GiantState &T = S[blockIdx.x];
+  const int32_t step0 = T.step;
+  __syncthreads();
+  if (step0 != GS_COUNT) return;
+  GrowVars V = T.V;
+  grow_size(V, (int64_t)T.ncube);
+  int32_t *hist = hist_all + (int64_t)blockIdx.x * HIST_SLICE;
+  for (int32_t b = threadIdx.x; b <= V.nb; b += GL_T) hist[b] = 0;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    T.V = V;
+    T.step = GS_HIST;
+    T.nch = part_chunks(V.ncube);
+  }
+}
+
+// One walk of the giants in phase `phase`, spread over the launch: work
+// items are (giant, chunk), taken from the counter *wq.  The count walk
+// takes chunks of CH_SEG segments, records each segment's particles in
+// segcnt and adds them to the giant's count.  Walks 1-3 take chunks of
+// about CH_PART particles: chunk k holds the segments whose first particle
+// (segpre, the prefix sums of segcnt) lies in [k CH_PART, (k+1) CH_PART),
+// so that the dense rows of a giant's core are spread over many
+// workgroups.  Walk 1 adds its LDS-counted inner bins to the giant's histogram with one
+// atomic per bin; walks 2 and 3 take slots of the giant's collection slice
+// with atomics on its fill counters; walk 3 adds its block sums to the
+// giant's sums.
+__global__ void __launch_bounds__(GL_T)
+k_giant_walk(SodCfg c, int32_t phase, const int32_t *__restrict__ cell_start,
+             const float4 *__restrict__ rec, const float *__restrict__ vel,
+             GiantState *__restrict__ S, int32_t ngiant,
+             int32_t *__restrict__ hist_all, uint64_t *__restrict__ pool_all,
+             int64_t pool_cap, int32_t *__restrict__ segcnt,
+             const int32_t *__restrict__ segpre, int32_t *__restrict__ wq) {
+  __shared__ GrowShared G;
+  __shared__ int32_t lh[HIST_LDS];
+  __shared__ int32_t s_g, s_k, s_q0, s_q1;
+  for (;;) {
     if (threadIdx.x == 0) {
-      if (failed) {
-        const int32_t k = atomicAdd(&counters[4], 1);
-        if (retry) retry[k] = e;
-        if (needmax) atomicMax(needmax, (unsigned long long)G.need);
+      int32_t k = atomicAdd(wq, 1), g = -1;
+      for (int32_t i = 0; i < ngiant; i++) {
+        if (S[i].step != phase) continue;
+        if (k < S[i].nch) {
+          g = i;
+          break;
+        }
+        k -= S[i].nch;
       }
-      if (res.status & PSOD_H_UNCLOSED) atomicAdd(&counters[5], 1);
-      store_result(out, h, c.nbins, res);
+      s_g = g;
+      s_k = k;
+    }
+    __syncthreads();
+    const int32_t g = s_g, k = s_k;
+    __syncthreads();
+    if (g < 0) break;
+    GiantState &T = S[g];
+    int64_t q0 = (int64_t)k * CH_SEG, q1 = q0 + CH_SEG;
+    if (phase != GS_COUNT) {
+      // the chunk's segments, from the giant's prefix sums
+      if (threadIdx.x < 64) {
+        const int32_t *pre = segpre + T.segoff;
+        const int32_t nseg = T.cseg;
+        const int32_t a = wave_lower_bound(pre, nseg, (int64_t)k * CH_PART);
+        const int32_t b =
+            k + 1 >= T.nch
+                ? nseg
+                : wave_lower_bound(pre, nseg, (int64_t)(k + 1) * CH_PART);
+        if (threadIdx.x == 0) {
+          s_q0 = a;
+          s_q1 = b;
+        }
+      }
+      __syncthreads();
+      q0 = s_q0;
+      q1 = s_q1;
+    }
+    const GrowVars V = T.V;
+    int32_t *hist = hist_all + (int64_t)g * HIST_SLICE;
+    uint64_t *pool = pool_all + (int64_t)g * pool_cap;
+    if (phase == GS_COUNT) {
+      const CellRef cr = V.cr;
+      const int64_t n =
+          cube_count(c, G, cell_start, cr, V.W, q0, q1, segcnt + T.segoff);
+      if (threadIdx.x == 0 && n > 0)
+        atomicAdd(&T.ncube, (unsigned long long)n);
+    } else if (phase == GS_HIST) {
+      const int32_t nl = min(V.nb, HIST_LDS);
+      for (int32_t b = threadIdx.x; b < nl; b += GL_T) lh[b] = 0;
+      __syncthreads();
+      walk_hist(c, G, lh, hist, cell_start, rec, V, q0, q1);
+      __syncthreads();
+      for (int32_t b = threadIdx.x; b < nl; b += GL_T)
+        if (lh[b]) atomicAdd(&hist[b], lh[b]);
+    } else {
+      for (int i = threadIdx.x; i < MAXRANGE; i += GL_T) {
+        G.rs[i] = T.rs[i];
+        G.re[i] = T.re[i];
+        G.off[i] = T.off[i];
+      }
+      if (phase == GS_PROFILE)
+        for (int j = threadIdx.x; j < c.nbins; j += GL_T) {
+          G.prad[j] = T.prad[j];
+          for (int w = 0; w < GL_T / 64; w++) G.pcnt[w][j] = 0;
+        }
+      __syncthreads();
+      if (phase == GS_COLLECT) {
+        walk_collect(c, G, cell_start, rec, V, T.nr, T.fill, pool, q0, q1);
+      } else {
+        const int flagv = (c.flags & PSOD_F_VELOCITY) && vel;
+        ProfileSums P = {{0, 0, 0}, {0, 0, 0}, 0};
+        walk_profile(c, G, cell_start, rec, vel, V, flagv, T.fill, pool, P,
+                     q0, q1);
+        P.nbd = block_sum_i64(P.nbd, G);
+        for (int d = 0; d < 3; d++) {
+          P.sp[d] = block_sum_f64(P.sp[d], G);
+          P.sv[d] = block_sum_f64(P.sv[d], G);
+        }
+        if (threadIdx.x == 0) {
+          if (P.nbd) atomicAdd(&T.nbd, (unsigned long long)P.nbd);
+          for (int d = 0; d < 3; d++) {
+            if (P.sp[d] != 0.0) atomicAdd(&T.sp[d], P.sp[d]);
+            if (P.sv[d] != 0.0) atomicAdd(&T.sv[d], P.sv[d]);
+          }
+        }
+        for (int j = threadIdx.x; j < c.nbins; j += GL_T) {
+          int32_t s = 0;
+          for (int w = 0; w < GL_T / 64; w++) s += G.pcnt[w][j];
+          if (s) atomicAdd(&T.pcnt[j], s);
+        }
+      }
     }
     __syncthreads();
   }
+}
+
+// bins for the giants in GS_HIST; clears the counter of the walk 2 launch
+__global__ void __launch_bounds__(GL_T)
+k_giant_bins(SodCfg c, GiantState *__restrict__ S,
+             int32_t *__restrict__ hist_all, int64_t pool_cap,
+             int32_t *__restrict__ counters, int32_t *__restrict__ retry,
+             unsigned long long *__restrict__ needmax, psod_halo_out out,
+             int32_t *__restrict__ wq) {
+  __shared__ GrowShared G;
+  __shared__ int32_t lh[HIST_LDS];
+  if (blockIdx.x == 0 && threadIdx.x == 0) wq[2] = 0;
+  GiantState &T = S[blockIdx.x];
+  const int32_t step0 = T.step;
+  __syncthreads();
+  if (step0 != GS_HIST) return;
+  GrowVars V = T.V;
+  giant_load(G, T);
+  int32_t *hist = hist_all + (int64_t)blockIdx.x * HIST_SLICE;
+  const int32_t step = grow_bins(c, G, lh, hist, V, pool_cap);
+  __syncthreads();
+  if (step == GS_FAILED) {
+    HaloResult res = T.res;
+    grow_close(c, counters, retry, needmax, G.need, V, res, true, out);
+  }
+  const int32_t nch = step == GS_COLLECT ? part_chunks(V.ncube) : 0;
+  giant_save(G, T, V, step, nch);
+}
+
+// cross for the giants in GS_COLLECT; clears the sums of walk 3 and the
+// counter of its launch
+__global__ void __launch_bounds__(GL_T)
+k_giant_cross(SodCfg c, GiantState *__restrict__ S,
+              const int32_t *__restrict__ hist_all,
+              uint64_t *__restrict__ pool_all, int64_t pool_cap,
+              const float *__restrict__ pos, int32_t *__restrict__ counters,
+              int32_t *__restrict__ retry,
+              unsigned long long *__restrict__ needmax, psod_halo_out out,
+              int32_t *__restrict__ wq) {
+  __shared__ GrowShared G;
+  if (blockIdx.x == 0 && threadIdx.x == 0) wq[3] = 0;
+  GiantState &T = S[blockIdx.x];
+  const int32_t step0 = T.step;
+  __syncthreads();
+  if (step0 != GS_COLLECT) return;
+  GrowVars V = T.V;
+  giant_load(G, T);
+  const int32_t *hist = hist_all + (int64_t)blockIdx.x * HIST_SLICE;
+  uint64_t *pool = pool_all + (int64_t)blockIdx.x * pool_cap;
+  HaloResult res = T.res;
+  double xcs[3] = {T.xcs[0], T.xcs[1], T.xcs[2]};
+  const int32_t step =
+      grow_cross(c, G, hist, pool, pos, V, res, xcs, pool_cap);
+  __syncthreads();
+  if (step == GS_FAILED)
+    grow_close(c, counters, retry, needmax, G.need, V, res, true, out);
+  if (threadIdx.x == 0) T.res = res;
+  if (step == GS_PROFILE) {
+    for (int j = threadIdx.x; j < 257; j += GL_T) T.pcnt[j] = 0;
+    if (threadIdx.x == 0) {
+      T.nbd = 0;
+      for (int d = 0; d < 3; d++) T.sp[d] = T.sv[d] = 0.0;
+    }
+  }
+  const int32_t nch = step == GS_PROFILE ? part_chunks(V.ncube) : 0;
+  giant_save(G, T, V, step, nch);
+}
+
+// finish for the giants in GS_PROFILE
+__global__ void __launch_bounds__(GL_T)
+k_giant_finish(SodCfg c, GiantState *__restrict__ S,
+               uint64_t *__restrict__ pool_all, int64_t pool_cap,
+               const float *__restrict__ pos, int32_t *__restrict__ counters,
+               int32_t *__restrict__ retry,
+               unsigned long long *__restrict__ needmax, psod_halo_out out) {
+  __shared__ GrowShared G;
+  GiantState &T = S[blockIdx.x];
+  const int32_t step0 = T.step;
+  __syncthreads();
+  if (step0 != GS_PROFILE) return;
+  GrowVars V = T.V;
+  giant_load(G, T);
+  for (int j = threadIdx.x; j < c.nbins; j += GL_T) G.pcnt[0][j] = T.pcnt[j];
+  ProfileSums P;
+  for (int d = 0; d < 3; d++) {
+    P.sp[d] = T.sp[d];
+    P.sv[d] = T.sv[d];
+  }
+  P.nbd = (int64_t)T.nbd;
+  __syncthreads();
+  uint64_t *pool = pool_all + (int64_t)blockIdx.x * pool_cap;
+  HaloResult res = T.res;
+  double xcs[3] = {T.xcs[0], T.xcs[1], T.xcs[2]};
+  grow_finish(c, G, pool, pos, V, res, xcs, P, out);
+  grow_close(c, counters, retry, needmax, 0, V, res, false, out);
+  if (threadIdx.x == 0) T.step = GS_DONE;
+}
+
+// giants still growing (not done, not failed) to *n
+__global__ void k_giant_count(const GiantState *__restrict__ S,
+                              int32_t ngiant, int32_t *__restrict__ n) {
+  if (blockIdx.x != 0 || threadIdx.x >= 64) return;
+  int32_t k = 0;
+  for (int32_t g = threadIdx.x; g < ngiant; g += 64)
+    k += S[g].step != GS_DONE && S[g].step != GS_FAILED;
+  k = wave_sum(k);
+  if (threadIdx.x == 0) *n = k;
 }
 
 // ---------------------------------------------------------------------------
@@ -2510,6 +3260,9 @@ struct SodScratch {
   int32_t *small, *large, *counters;
   int32_t *ntask, *task0, *ntot, *retry;
   unsigned long long *hbest;
+  int32_t *giants;     // (e, W) of the giant halos left to the staged growth
+  int64_t *glay;       // their per-segment count layout (k_giant_layout)
+  GiantState *gstate;  // their states, GIANT_CAP entries
 };
 
 static int sod_scratch(psod_ctx *ctx, int64_t nhalo, SodScratch *S) {
@@ -2521,7 +3274,12 @@ static int sod_scratch(psod_ctx *ctx, int64_t nhalo, SodScratch *S) {
   const int32_t ncap = (int32_t)nhalo;
   int32_t *ntask = (int32_t *)psod_scratch(
       ctx, S_MISC, (size_t)(5 * ncap + 128) * 4 + (size_t)ncap * 16 + 512);
-  if (!S->plan || !lists || !ntask) return PSOD_E_NOMEM;
+  char *gb = (char *)psod_scratch(
+      ctx, S_GIANT, 1536 + (size_t)GIANT_CAP * sizeof(GiantState));
+  if (!S->plan || !lists || !ntask || !gb) return PSOD_E_NOMEM;
+  S->giants = (int32_t *)gb;            // 2 * GIANT_CAP entries
+  S->glay = (int64_t *)(gb + 512);      // GIANT_CAP + 1 entries
+  S->gstate = (GiantState *)(gb + 1536);
   S->small = lists;
   S->large = lists + nhalo;
   S->counters = lists + 2 * nhalo;
@@ -2581,7 +3339,7 @@ static int grow_retries(psod_ctx *ctx, const SodCfg &c,
     k_grow_large<<<ngrow2, GL_T, 0, st>>>(
         c, cat->cell_start, (const float4 *)cat->rec, cat->pos, cat->vel,
         cat->ids, S.plan, S.large, c2, hist, pool2, cap2, cur, nxt, need2,
-        nullptr, *out);
+        nullptr, nullptr, 0, *out);
     PSOD_LAUNCH_CHECK();
     int32_t hf2[2];
     PSOD_HIP(hipMemcpyAsync(hf2, c2 + 4, 8, hipMemcpyDeviceToHost, st));
@@ -2625,6 +3383,7 @@ static int sod_fof_queue(psod_ctx *ctx, const psod_sod_params *p,
   c.small_max = p->small_max > 0 ? std::min(p->small_max, CAP5)
                                  : std::min(tun.small_max, CAP5);
   c.bound_min = tun.bound_min;
+  c.giant_min = tun.giant_min;
   hipStream_t st = ctx->stream;
   PSOD_TRY(sod_scratch(ctx, nhalo, S));
   HaloPlan *plan = S->plan;
@@ -2667,7 +3426,7 @@ static int sod_fof_queue(psod_ctx *ctx, const psod_sod_params *p,
     return PSOD_OK;
   };
   if (!dev)
-    for (int i = 0; i < 10; i++) ctx->sod_stat[i] = 0;
+    for (int i = 0; i < 12; i++) ctx->sod_stat[i] = 0;
   // largest halos first (centre-search tasks and growth), so that the
   // persistent workgroups do not finish on a giant halo picked up last
   auto size_order = [&](int32_t n, const int32_t *dn, int growth) -> int {
@@ -2713,18 +3472,108 @@ static int sod_fof_queue(psod_ctx *ctx, const psod_sod_params *p,
                                             (size_t)pool_cap * 8 * ngrow);
   if (!hist || !pool) return PSOD_E_NOMEM;
   // counters[16..17]: largest slice a failed halo needs; counters[24..31]
-  // and [32..33]: the same for a re-run launch
+  // and [32..33]: the same for a re-run launch; counters[6]: giant halos
+  // left to the staged growth
   unsigned long long *needmax = (unsigned long long *)(counters + 16);
   k_grow_large<<<ngrow, GL_T, 0, st>>>(
       c, cat->cell_start, (const float4 *)cat->rec, cat->pos, cat->vel,
       cat->ids, plan, large, counters, hist, pool, pool_cap, nullptr,
-      S->retry, needmax, dev ? counters + 36 : nullptr, *out);
+      S->retry, needmax, dev ? counters + 36 : nullptr, S->giants, GIANT_CAP,
+      *out);
   PSOD_LAUNCH_CHECK();
   return PSOD_OK;
 }
 
-// After the first growth launch: reads the failures back, runs the re-run
-// launches they need and reports unclosed halos.  nlarge is the large-halo
+// Staged growth of the giant halos that the first growth launch listed:
+// cycles of the k_giant_* launches until every giant is done or failed
+// (failures join the re-run list of the first launch).  A cycle ends with
+// one read-back of the giants still growing; a giant takes a second cycle
+// only when its crossing lies beyond its cube.
+static int grow_giants(psod_ctx *ctx, const SodCfg &c,
+                       const psod_catalogue *cat, const SodScratch &S,
+                       int32_t ngiant, const psod_halo_out *out) {
+  hipStream_t st = ctx->stream;
+  const int64_t pool_cap = growth_pool_cap();
+  int32_t *hist = (int32_t *)psod_scratch(ctx, S_GIANT_H,
+                                          (size_t)ngiant * HIST_SLICE * 4);
+  uint64_t *pool = (uint64_t *)psod_scratch(ctx, S_GIANT_P,
+                                            (size_t)ngiant * pool_cap * 8);
+  if (!hist || !pool) return PSOD_E_NOMEM;
+  GiantState *gs = S.gstate;
+  int32_t *wq = S.counters + 8;      // counters[8..11]: the walks' queues
+  int32_t *nleft = S.counters + 12;  // giants still growing
+  unsigned long long *needmax = (unsigned long long *)(S.counters + 16);
+  const float4 *rec = (const float4 *)cat->rec;
+  // the walk kernel holds one 512-thread workgroup per CU (its LDS)
+  const unsigned nwalk = (unsigned)ctx->num_cus;
+  k_giant_init<<<ngiant, 64, 0, st>>>(c, cat->pos, cat->ids, S.plan, S.large,
+                                      S.giants, gs, wq);
+  PSOD_LAUNCH_CHECK();
+  int32_t *segcnt = nullptr, *segpre = nullptr;
+  auto walk = [&](int32_t phase, int32_t *q) -> int {
+    k_giant_walk<<<nwalk, GL_T, 0, st>>>(c, phase, cat->cell_start, rec,
+                                         cat->vel, gs, ngiant, hist, pool,
+                                         pool_cap, segcnt, segpre, q);
+    PSOD_LAUNCH_CHECK();
+    return PSOD_OK;
+  };
+  constexpr int MAX_CYCLES = 16;  // the reach at least doubles per cycle
+  int cyc = 0;
+  for (;; cyc++) {
+    if (cyc == MAX_CYCLES)
+      return psod_set_error(PSOD_E_UNCLOSED,
+                            "sod: giant halos still growing after %d cycles",
+                            MAX_CYCLES);
+    k_giant_begin<<<ngiant, GL_T, 0, st>>>(c, gs, S.counters, S.retry,
+                                           needmax, *out, wq);
+    PSOD_LAUNCH_CHECK();
+    // per-segment counts of the cubes and their prefix sums, per giant
+    k_giant_layout<<<1, 64, 0, st>>>(gs, ngiant, S.glay);
+    PSOD_LAUNCH_CHECK();
+    int64_t lay[GIANT_CAP + 1];
+    PSOD_HIP(hipMemcpyAsync(lay, S.glay, (size_t)(ngiant + 1) * 8,
+                            hipMemcpyDeviceToHost, st));
+    PSOD_HIP(hipStreamSynchronize(st));
+    segcnt = (int32_t *)psod_scratch(ctx, S_GIANT_SEG,
+                                     (size_t)(2 * lay[ngiant] + 64) * 4);
+    if (!segcnt) return PSOD_E_NOMEM;
+    segpre = segcnt + lay[ngiant] + 32;
+    PSOD_TRY(walk(GS_COUNT, wq + 0));
+    for (int32_t g = 0, off = 0; g < ngiant; off += (int32_t)lay[g], g++)
+      if (lay[g] > 0)
+        PSOD_TRY(psod_scan_i32(ctx, segcnt + off, segpre + off, lay[g],
+                               nullptr));
+    k_giant_size<<<ngiant, GL_T, 0, st>>>(c, gs, hist, wq);
+    PSOD_LAUNCH_CHECK();
+    PSOD_TRY(walk(GS_HIST, wq + 1));
+    k_giant_bins<<<ngiant, GL_T, 0, st>>>(c, gs, hist, pool_cap, S.counters,
+                                          S.retry, needmax, *out, wq);
+    PSOD_LAUNCH_CHECK();
+    PSOD_TRY(walk(GS_COLLECT, wq + 2));
+    k_giant_cross<<<ngiant, GL_T, 0, st>>>(c, gs, hist, pool, pool_cap,
+                                           cat->pos, S.counters, S.retry,
+                                           needmax, *out, wq);
+    PSOD_LAUNCH_CHECK();
+    PSOD_TRY(walk(GS_PROFILE, wq + 3));
+    k_giant_finish<<<ngiant, GL_T, 0, st>>>(c, gs, pool, pool_cap, cat->pos,
+                                            S.counters, S.retry, needmax,
+                                            *out);
+    PSOD_LAUNCH_CHECK();
+    k_giant_count<<<1, 64, 0, st>>>(gs, ngiant, nleft);
+    PSOD_LAUNCH_CHECK();
+    int32_t left = 0;
+    PSOD_HIP(hipMemcpyAsync(&left, nleft, 4, hipMemcpyDeviceToHost, st));
+    PSOD_HIP(hipStreamSynchronize(st));
+    if (left == 0) break;
+  }
+  ctx->sod_stat[10] = ngiant;
+  ctx->sod_stat[11] = cyc + 1;
+  return PSOD_OK;
+}
+
+// After the first growth launch: runs the staged growth of the giant
+// halos, reads the failures back, runs the re-run launches they need and
+// reports unclosed halos.  nlarge is the large-halo
 // count (read back here when the pass ran with device counts).
 static int sod_finish_growth(psod_ctx *ctx, const psod_sod_params *p,
                              const psod_catalogue *cat, int64_t nhalo,
@@ -2753,10 +3602,20 @@ static int sod_finish_growth(psod_ctx *ctx, const psod_sod_params *p,
   c.small_max = p->small_max > 0 ? std::min(p->small_max, CAP5)
                                  : std::min(tun.small_max, CAP5);
   c.bound_min = tun.bound_min;
+  c.giant_min = tun.giant_min;
   const int32_t ngrow = std::min<int32_t>(nlarge, ctx->num_cus);
   int32_t *hist = (int32_t *)psod_scratch(ctx, S_BINS,
                                           (size_t)ngrow * HIST_SLICE * 4);
   if (!hist) return PSOD_E_NOMEM;
+  const int32_t ngiant = std::min(hc[6], GIANT_CAP);
+  if (ngiant > 0) {
+    PSOD_TRY(grow_giants(ctx, c, cat, S, ngiant, out));
+    // the staged giants' failures and unclosed halos join the counts
+    PSOD_HIP(hipMemcpyAsync(hc, S.counters, 32, hipMemcpyDeviceToHost, st));
+    PSOD_HIP(hipMemcpyAsync(&hneed, S.counters + 16, 8,
+                            hipMemcpyDeviceToHost, st));
+    PSOD_HIP(hipStreamSynchronize(st));
+  }
   int32_t unclosed = hc[5];
   PSOD_TRY(grow_retries(ctx, c, cat, S, nlarge, ngrow, hist,
                         growth_pool_cap(), hc[4], hneed, &unclosed, out));

@@ -24,6 +24,7 @@ struct SodCfg {
   // tunables: one-wave halo cap (particles in the 5^3 cells) and the size
   // from which the centre search bounds its boxes first
   int32_t small_max, bound_min;
+  int64_t giant_min;  // growth cube particles from which growth is staged
 };
 
 // overdensity of the k closest particles at radius r, in the reference's

@@ -27,7 +27,7 @@ static int32_t clamp_or(int32_t v, int32_t def, int32_t lo, int32_t hi) {
 
 psod_tunables psod_tunables_for(const psod_ctx *c, int32_t nx) {
   psod_tunables t = {TUNE_SMALL_MAX, TUNE_BOUND_MIN, TUNE_DENSE_TILE,
-                     TUNE_SMALL_WG};
+                     TUNE_SMALL_WG, TUNE_GIANT_MIN};
   for (int i = 0; i < c->ntune_file; i++)
     if (c->tune_file[i].nx == nx) {
       const psod_tunables &f = c->tune_file[i].t;
@@ -35,23 +35,27 @@ psod_tunables psod_tunables_for(const psod_ctx *c, int32_t nx) {
       if (f.bound_min > 0) t.bound_min = f.bound_min;
       if (f.dense_tile > 0) t.dense_tile = f.dense_tile;
       if (f.small_wg_per_cu > 0) t.small_wg_per_cu = f.small_wg_per_cu;
+      if (f.giant_min > 0) t.giant_min = f.giant_min;
     }
   const psod_tunables &s = c->tune_set;
   if (s.small_max > 0) t.small_max = s.small_max;
   if (s.bound_min > 0) t.bound_min = s.bound_min;
   if (s.dense_tile > 0) t.dense_tile = s.dense_tile;
   if (s.small_wg_per_cu > 0) t.small_wg_per_cu = s.small_wg_per_cu;
+  if (s.giant_min > 0) t.giant_min = s.giant_min;
   t.small_max = clamp_or(t.small_max, TUNE_SMALL_MAX, 64, TUNE_SMALL_MAX);
   t.bound_min = clamp_or(t.bound_min, TUNE_BOUND_MIN, 1, 1 << 30);
   t.dense_tile = clamp_or(t.dense_tile, TUNE_DENSE_TILE, 1, 1 << 30);
   t.small_wg_per_cu = clamp_or(t.small_wg_per_cu, TUNE_SMALL_WG, 1, 32);
+  t.giant_min = clamp_or(t.giant_min, TUNE_GIANT_MIN, 1, 1 << 30);
   return t;
 }
 
 extern "C" int psod_set_tunables(psod_ctx *ctx, const psod_tunables *t) {
   if (!ctx || !t) return psod_set_error(PSOD_E_ARG, "psod_set_tunables: null");
   if (t->small_max < 0 || t->small_max > TUNE_SMALL_MAX || t->bound_min < 0 ||
-      t->dense_tile < 0 || t->small_wg_per_cu < 0 || t->small_wg_per_cu > 32)
+      t->dense_tile < 0 || t->small_wg_per_cu < 0 ||
+      t->small_wg_per_cu > 32 || t->giant_min < 0)
     return psod_set_error(PSOD_E_ARG, "psod_set_tunables: out of range "
                           "(small_max %d <= %d, small_wg_per_cu %d <= 32)",
                           t->small_max, TUNE_SMALL_MAX, t->small_wg_per_cu);
@@ -76,11 +80,11 @@ extern "C" int psod_load_tunables(psod_ctx *ctx, const char *path,
   int n = 0;
   while (fgets(line, sizeof(line), f)) {
     char tag[32], dev[256];
-    int cus, nx, a, b, cc, d;
+    int cus, nx, a, b, cc, d, e;
     long long np, nh;
     double ms;
-    if (sscanf(line, "%31s %255s %d %d %lld %lld %d %d %d %d %lf", tag, dev,
-               &cus, &nx, &np, &nh, &a, &b, &cc, &d, &ms) != 11)
+    if (sscanf(line, "%31s %255s %d %d %lld %lld %d %d %d %d %d %lf", tag,
+               dev, &cus, &nx, &np, &nh, &a, &b, &cc, &d, &e, &ms) != 12)
       continue;
     if (strcmp(tag, "psod-tune") != 0 || strcmp(dev, ctx->device_name) != 0 ||
         cus != ctx->num_cus)
@@ -93,7 +97,7 @@ extern "C" int psod_load_tunables(psod_ctx *ctx, const char *path,
       slot = ctx->ntune_file++;
     }
     ctx->tune_file[slot].nx = nx;
-    ctx->tune_file[slot].t = psod_tunables{a, b, cc, d};
+    ctx->tune_file[slot].t = psod_tunables{a, b, cc, d, e};
     n++;
   }
   fclose(f);
@@ -217,14 +221,18 @@ extern "C" int psod_autotune(psod_ctx *ctx, const psod_sod_params *p,
   // SOD: one-wave cap, bound threshold, one-wave workgroups per CU
   double best_sod;
   PSOD_TRY(time_sod(best, &best_sod));
-  for (int k = 0; k < 3; k++) {
-    static const int32_t cand[3][5] = {{256, 512, 768, 1024, 0},
+  for (int k = 0; k < 4; k++) {
+    static const int32_t cand[4][5] = {{256, 512, 768, 1024, 0},
                                        {512, 1024, 2048, 4096, 8192},
-                                       {4, 8, 16, 0, 0}};
+                                       {4, 8, 16, 0, 0},
+                                       {1 << 19, 1 << 20, 1 << 21, 1 << 22,
+                                        1 << 30}};
     for (int j = 0; j < 5 && cand[k][j] > 0; j++) {
       psod_tunables t = best;
-      (k == 0 ? t.small_max : k == 1 ? t.bound_min : t.small_wg_per_cu) =
-          cand[k][j];
+      (k == 0   ? t.small_max
+       : k == 1 ? t.bound_min
+       : k == 2 ? t.small_wg_per_cu
+                : t.giant_min) = cand[k][j];
       double ms;
       PSOD_TRY(time_sod(t, &ms));
       if (ms < best_sod) {
@@ -247,10 +255,10 @@ extern "C" int psod_autotune(psod_ctx *ctx, const psod_sod_params *p,
     if (!f)
       return psod_set_error(PSOD_E_ARG, "psod_autotune: cannot append to %s",
                             cache_file);
-    fprintf(f, "psod-tune %s %d %d %lld %lld %d %d %d %d %.3f\n",
+    fprintf(f, "psod-tune %s %d %d %lld %lld %d %d %d %d %d %.3f\n",
             ctx->device_name, ctx->num_cus, nx, (long long)npart,
             (long long)nhalo, best.small_max, best.bound_min, best.dense_tile,
-            best.small_wg_per_cu, best_dep + best_sod);
+            best.small_wg_per_cu, best.giant_min, best_dep + best_sod);
     fclose(f);
   }
   if (chosen) *chosen = best;

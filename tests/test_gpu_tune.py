@@ -25,11 +25,12 @@ def test_tunables_do_not_change_results(ctx):
     nx = p.nx
     base = ctx.tunables(nx)
     assert base == dict(small_max=1024, bound_min=2048, dense_tile=16384,
-                        small_wg_per_cu=8)
+                        small_wg_per_cu=8, giant_min=1 << 21)
     runs = []
     for t in (dict(), dict(small_max=256, bound_min=64, dense_tile=512,
-                           small_wg_per_cu=2),
-              dict(small_max=768, bound_min=1 << 20, small_wg_per_cu=32)):
+                           small_wg_per_cu=2, giant_min=2000),
+              dict(small_max=768, bound_min=1 << 20, small_wg_per_cu=32,
+                   giant_min=1)):
         ctx.set_tunables(**t)
         grid = ctx._empty((nx, nx, nx), ctx.torch.float32)
         o = ctx.run_pass(p, pos, lab, nh, vel=vel, ids=ids, grid=grid)
@@ -74,13 +75,13 @@ def test_autotune_file_round_trip(ctx, tmp_path):
     assert ctx.tunables(p.nx) == chosen
     line = open(f).read().split()
     assert line[0] == "psod-tune" and int(line[3]) == p.nx
-    assert [int(x) for x in line[6:10]] == [
+    assert [int(x) for x in line[6:11]] == [
         chosen["small_max"], chosen["bound_min"], chosen["dense_tile"],
-        chosen["small_wg_per_cu"]]
+        chosen["small_wg_per_cu"], chosen["giant_min"]]
     # a tuning file for another grid size leaves this one at its defaults
     with open(f, "a") as fh:
         fh.write(" ".join(line[:3] + ["128"] + line[4:6] +
-                          ["512", "4096", "8192", "4", "1.0"]) + "\n")
+                          ["512", "4096", "8192", "4", "65536", "1.0"]) + "\n")
     old = os.environ.get("PSOD_TUNE_FILE")
     os.environ["PSOD_TUNE_FILE"] = f
     try:
@@ -93,7 +94,42 @@ def test_autotune_file_round_trip(ctx, tmp_path):
     try:
         assert c2.tunables(p.nx) == chosen
         assert c2.tunables(128) == dict(small_max=512, bound_min=4096,
-                                        dense_tile=8192, small_wg_per_cu=4)
+                                        dense_tile=8192, small_wg_per_cu=4,
+                                        giant_min=65536)
         assert c2.tunables(32)["small_max"] == 1024
     finally:
         c2.close()
+
+
+def test_staged_growth_of_giant_halos(ctx):
+    """A 128^3 catalogue with a 1.5e6-particle central clump: halos grown
+    by the staged launches (giant_min at or below their cubes; at 4096 more
+    halos qualify than the 64 staged per pass, so the rest stay on one
+    workgroup each) give the results of one workgroup per halo (giant_min
+    above every cube)."""
+    import psod_amd as P
+    nx = 128
+    N = nx ** 3
+    g = P.gen_params(N, 1500, nlo=20, nhi=20000, seed=17,
+                     ncentral=1_500_000, ccentral=8.0)
+    pos, vel, lab, ids, tab = ctx.generate(g)
+    nh = len(tab["size"])
+    p = P.sod_params(nx, N, deltas=(200.0, 500.0), delta2=800.0,
+                     velocity=True)
+    runs = []
+    for gm in (1 << 30, 1 << 20, 4096):
+        ctx.set_tunables(giant_min=gm)
+        o = ctx.run_pass(p, pos, lab, nh, vel=vel, ids=ids)
+        runs.append({k: v.cpu().numpy() for k, v in o.items()})
+    ctx.set_tunables()
+    o0 = runs[0]
+    assert np.all(o0["status"] & 1)
+    assert o0["count"][:, 0].max() > 500_000
+    for o in runs[1:]:
+        for k in ("centre", "centre_id", "status", "w", "count", "radius",
+                  "count2", "profile", "com_pos", "ngather", "nboundary"):
+            np.testing.assert_array_equal(o[k], o0[k], err_msg=k)
+        for k in ("rint", "mint", "overdens", "overdens2"):
+            np.testing.assert_array_equal(o[k], o0[k], err_msg=k)
+        np.testing.assert_allclose(o["com_vel"], o0["com_vel"], rtol=1e-12,
+                                   atol=1e-12)
