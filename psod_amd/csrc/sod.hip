@@ -1856,11 +1856,12 @@ __device__ void block_sort_pool(uint64_t *pool, int32_t n, GrowShared &G) {
 // bins and cross send the halo back to begin with a larger reach when the
 // crossing lies beyond the cube, or fail it when its bands exceed the
 // collection slice (a re-run launch then grows it with larger slices).
-// k_grow_large runs the phases of each halo in one workgroup.  A giant halo
-// (cube of at least c.giant_min particles) is left to the staged launches
-// k_giant_*: one workgroup per giant halo for begin, bins, cross and
-// finish, every workgroup of the GPU for its walks, each phase a launch of
-// its own with the halo's state kept in global memory between launches.
+// k_grow_large runs the same steps inline, each halo in one workgroup.  A
+// giant halo (cube of at least c.giant_min particles) is left to the staged
+// launches k_giant_*, which call the phase functions below: one workgroup
+// per giant halo for begin, bins, cross and finish, every workgroup of the
+// GPU for its walks, each phase a launch of its own with the halo's state
+// kept in global memory between launches.
 // ---------------------------------------------------------------------------
 enum : int32_t {
   GS_BEGIN = 0,
@@ -2518,6 +2519,12 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
   // `giants`, a halo whose cube reaches c.giant_min particles is appended
   // there as (e, W) instead (counted in counters[6], at most giant_cap
   // entries) and left to the staged launches.
+  //
+  // The phases of the growth run here inline, in one function body: the
+  // same steps called as the phase functions above (as the staged launches
+  // call them) made this kernel 12% slower at 1024^3 (38 against 33.5 ms,
+  // same instruction counts, more wave cycles waiting).  The two forms are
+  // held to identical results by the tests of the staged growth.
   __shared__ GrowShared G;
   __shared__ int32_t lh[HIST_LDS];  // inner bins; staging of the scan
   // a captured pass whose centre search overflowed: the host runs the pass
@@ -2534,24 +2541,37 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
     const int32_t item = G.item;
     __syncthreads();
     if (item >= nitem) break;
-    GrowVars V;
+    const int32_t e = elist ? elist[item] : item;
+    const int64_t h = large[e];
+    const HaloPlan pl = plan[h];
+    const int32_t centre = pl.centre;
+    const CellRef cr = ref_of_particle(pos, centre, c.nx);
     HaloResult res;
-    double xcs[3];
-    grow_start(c, pos, ids, plan, large, elist ? elist[item] : item, V, res,
-               xcs);
-    if (threadIdx.x == 0) G.bump = 0;
-    int32_t step = GS_BEGIN;
+    zero_result(res);
+    res.centre = centre;
+    res.centre_id = ids ? ids[centre] : (int64_t)centre;
+    int32_t W = pl.w0 < 2 ? 2 : pl.w0;
+    bool failed = false;
     bool deferred = false;
-    while (step == GS_BEGIN) {
-      step = grow_begin(c, G, cell_start, V, res);
-      if (step != GS_HIST) break;
-      if (giants && V.ncube >= c.giant_min) {
+    if (threadIdx.x == 0) G.bump = 0;
+    double xcs[3];
+    centre_scaled(pos, centre, c.nx, xcs);
+    for (;;) {
+      if (2 * W + 1 > c.nx) {
+        res.status = PSOD_H_UNCLOSED | PSOD_H_LARGE;
+        res.w = W;
+        break;
+      }
+      const float W2 = (float)W * (float)W;
+      // bins uniform in r^3, about 4 particles of the cube per bin
+      const int64_t ncube = cube_count(c, G, cell_start, cr, W);
+      if (giants && ncube >= c.giant_min) {
         if (threadIdx.x == 0) {
           const int32_t k = atomicAdd(&counters[6], 1);
           G.item = k < giant_cap ? 1 : 0;
           if (k < giant_cap) {
-            giants[2 * k] = V.e;
-            giants[2 * k + 1] = V.W;
+            giants[2 * k] = e;
+            giants[2 * k + 1] = W;
           }
         }
         __syncthreads();
@@ -2559,43 +2579,444 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
         __syncthreads();
         if (deferred) break;
       }
-      const int32_t nl = min(V.nb, HIST_LDS);
+      int32_t nb = HIST_MIN;
+      while ((int64_t)nb * 4 < ncube && nb < HIST_MAX) nb <<= 1;
+      const float scale = (float)nb / ((float)W * (float)W * (float)W);
+      const int32_t nl = min(nb, HIST_LDS);
       for (int32_t b = threadIdx.x; b < nl; b += GL_T) lh[b] = 0;
-      for (int32_t b = nl + threadIdx.x; b <= V.nb; b += GL_T) hist[b] = 0;
+      for (int32_t b = nl + threadIdx.x; b <= nb; b += GL_T) hist[b] = 0;
       __threadfence_block();
       __syncthreads();
-      walk_hist(c, G, lh, hist, cell_start, rec, V);
+      // ---- pass 1: histogram -------------------------------------------
+      // every particle of the cube is gathered; those of d2 < W2 are binned
+      const int64_t ng = ncube;
+      cube_walk(c, G, cell_start, rec, cr, W, -1.0f,
+                W2 * (1.0f + 1e-4f) + 1e-3f,
+                [&](float d2, int32_t, const float4 &, int, int, int,
+                    const int32_t *) {
+                  if (d2 < W2) {
+                    const int32_t b = rbin(d2, scale, nb);
+                    if (b < HIST_LDS) atomicAdd(&lh[b], 1);
+                    else atomicAdd(&hist[b], 1);
+                  }
+                });
       for (int32_t b = threadIdx.x; b < nl; b += GL_T) hist[b] = lh[b];
       __threadfence();
       __syncthreads();
-      step = grow_bins(c, G, lh, hist, V, pool_cap);
-      if (step != GS_COLLECT) continue;
-      walk_collect(c, G, cell_start, rec, V, G.nr, G.fill, pool);
-      __threadfence();
-      __syncthreads();
-      step = grow_cross(c, G, hist, pool, pos, V, res, xcs, pool_cap);
-      if (step != GS_PROFILE) continue;
-      const int flagv = (c.flags & PSOD_F_VELOCITY) && vel;
-      ProfileSums S = {{0, 0, 0}, {0, 0, 0}, 0};
-      walk_profile(c, G, cell_start, rec, vel, V, flagv, G.fill, pool, S);
-      S.nbd = block_sum_i64(S.nbd, G);
-      for (int d = 0; d < 3; d++) {
-        S.sp[d] = block_sum_f64(S.sp[d], G);
-        S.sv[d] = block_sum_f64(S.sv[d], G);
+      // exclusive scan in place: hist[b] = #(bin < b), hist[nb] = total.
+      // Rounds of SCAN_ROUND bins, read coalesced into LDS; thread t scans
+      // the SCAN_ROUND / GL_T consecutive bins from t * SCAN_ROUND / GL_T.
+      {
+        constexpr int PER = SCAN_ROUND / GL_T;
+        const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+        int32_t carry = 0;
+        for (int32_t base = 0; base < nb; base += SCAN_ROUND) {
+          for (int i = threadIdx.x; i < SCAN_ROUND; i += GL_T)
+            lh[i] = ldcg(&hist[base + i]);
+          __syncthreads();
+          int32_t v[PER], sum = 0;
+#pragma unroll
+          for (int i = 0; i < PER; i++) {
+            v[i] = lh[threadIdx.x * PER + i];
+            sum += v[i];
+          }
+          int32_t tot;
+          const int32_t ex = wave_excl(sum, &tot);
+          if (l == 0) G.red_i[w] = tot;
+          __syncthreads();
+          int32_t run = carry + ex, all = 0;
+          for (int i = 0; i < GL_T / 64; i++) {
+            if (i < w) run += G.red_i[i];
+            all += G.red_i[i];
+          }
+#pragma unroll
+          for (int i = 0; i < PER; i++) {
+            lh[threadIdx.x * PER + i] = run;
+            run += v[i];
+          }
+          __syncthreads();
+          for (int i = threadIdx.x; i < SCAN_ROUND; i += GL_T)
+            hist[base + i] = lh[i];
+          carry += all;
+          __syncthreads();
+        }
+        if (threadIdx.x == 0) hist[nb] = carry;
+        __threadfence();
+        __syncthreads();
       }
-      // per-bin counts of all waves in G.pcnt[0]
+      const int32_t *C = hist;
+      const double Wd = (double)W, W3 = Wd * Wd * Wd;
+      // ---- bins that may hold each threshold's first crossing --------------
+      // A bin is possible for delta when the smallest admissible rank k in it
+      // could have od_k <= delta at the bin's outer edge, and definite when
+      // it must at the inner edge; the crossing lies in a possible bin no
+      // later than the first definite one.  The possible bins are collected
+      // as up to SEGCAP segments, each extended down to the previous
+      // non-empty bin, which holds rank k-1 when k opens the segment.
+      bool ok = true;
+      auto bin_k = [&](int32_t b, int64_t *kmin) {
+        const int32_t cb = ldcg(&C[b]), ce = ldcg(&C[b + 1]);
+        *kmin = max((int64_t)cb + 1, (int64_t)c.nmin_count);
+        return *kmin <= ce;
+      };
+      auto nonempty = [&](int32_t b) { return ldcg(&C[b + 1]) > ldcg(&C[b]); };
+      for (int di = 0; di < c.ndelta && ok; di++) {
+        const double dlt = c.delta[di];
+        auto definite = [&](int32_t b) {
+          int64_t k;
+          return bin_k(b, &k) && rad3_for(k, c.mpart, dlt) <= edge3_lo(b, nb, W3);
+        };
+        auto possible = [&](int32_t b) {
+          int64_t k;
+          return bin_k(b, &k) && rad3_for(k, c.mpart, dlt) < edge3_hi(b, nb, W3);
+        };
+        const int32_t b2 = block_first(0, nb - 1, definite, G);
+        if (b2 == INT32_MAX) { ok = false; break; }
+        int32_t last_e = -1, x = 0, ns = 0;
+        for (;;) {
+          const int32_t s0 = block_first(x, b2, possible, G);
+          int32_t e = b2;
+          if (ns < SEGCAP - 1) {
+            const int32_t q = block_first(
+                s0 + 1, b2, [&](int32_t b) { return nonempty(b) && !possible(b); },
+                G);
+            if (q != INT32_MAX) e = q - 1;
+          }
+          // a run close behind the previous segment (few keys between)
+          // extends it instead of opening a segment
+          const bool join =
+              ns > 0 && ldcg(&C[s0]) - ldcg(&C[last_e + 1]) <= GAP_KEYS;
+          const int32_t xs = (!join && s0 > last_e + 1)
+                                 ? block_last(last_e + 1, s0 - 1, nonempty, G)
+                                 : -1;
+          if (threadIdx.x == 0) {
+            if (join) {
+              G.sg_e[di][ns - 1] = e;
+            } else {
+              G.sg_s[di][ns] = s0;
+              G.sg_e[di][ns] = e;
+              G.sg_x[di][ns] = xs >= 0 ? xs : s0;
+            }
+          }
+          if (!join) ns++;
+          if (e >= b2) break;
+          last_e = e;
+          x = e + 1;
+        }
+        if (threadIdx.x == 0) G.nseg[di] = ns;
+        __syncthreads();
+      }
+      if (!ok) {
+        W = 2 * W;
+        __syncthreads();
+        continue;
+      }
+      // ---- disjoint ranges of bins to collect -------------------------------
+      if (threadIdx.x == 0) {
+        int32_t nr = 0;
+        for (int di = 0; di < c.ndelta; di++)
+          for (int j = 0; j < G.nseg[di]; j++) {
+            int32_t a = G.sg_x[di][j], t = G.sg_e[di][j];
+            // absorb every range that overlaps or touches [a, t]
+            int32_t w = 0;
+            for (int r = 0; r < nr; r++) {
+              if (!(t < G.rs[r] - 1 || a > G.re[r] + 1)) {
+                a = min(a, G.rs[r]);
+                t = max(t, G.re[r]);
+              } else {
+                G.rs[w] = G.rs[r];
+                G.re[w] = G.re[r];
+                w++;
+              }
+            }
+            G.rs[w] = a;
+            G.re[w] = t;
+            nr = w + 1;
+          }
+        // ascending order of the ranges (ranks then ascend with them)
+        for (int r = 1; r < nr; r++)
+          for (int q = r; q > 0 && G.rs[q - 1] > G.rs[q]; q--) {
+            const int32_t ts = G.rs[q], te = G.re[q];
+            G.rs[q] = G.rs[q - 1];
+            G.re[q] = G.re[q - 1];
+            G.rs[q - 1] = ts;
+            G.re[q - 1] = te;
+          }
+        G.nr = nr;
+        int64_t oo = 0;
+        for (int r = 0; r < nr; r++) {
+          G.k0[r] = ldcg(&C[G.rs[r]]);
+          G.k1[r] = ldcg(&C[G.re[r] + 1]);
+          G.off[r] = oo;
+          G.fill[r] = 0;
+          oo += pool_slots(G.k1[r] - G.k0[r]);
+        }
+        G.bump = oo;  // a re-run starts the slice afresh
+        G.need = oo;
+      }
+      __syncthreads();
+      const int nr = G.nr;
+      if (G.need > pool_cap) { failed = true; break; }
+      // ---- pass 2: collect the ranges ---------------------------------------
+      float blo2 = -1.0f, bhi2 = -2.0f;  // nothing when there is no range
+      if (nr > 0) bin_band(G.rs[0], G.re[nr - 1], nb, scale, W2, &blo2, &bhi2);
+      cube_walk(c, G, cell_start, rec, cr, W, blo2, bhi2,
+                [&](float d2, int32_t idx, const float4 &, int, int, int,
+                    const int32_t *) {
+                  if (!(d2 < W2)) return;
+                  const int32_t b = rbin(d2, scale, nb);
+                  for (int r = 0; r < nr; r++)
+                    if (b >= G.rs[r] && b <= G.re[r]) {
+                      const int32_t s = atomicAdd(&G.fill[r], 1);
+                      pool[G.off[r] + s] = d2key(d2, idx);
+                    }
+                });
+      __threadfence_block();
+      __syncthreads();
+      for (int r = 0; r < nr; r++)
+        block_sort_pool(pool + G.off[r], (int32_t)(G.k1[r] - G.k0[r]), G);
+      // key of rank k (1-based) among the collected ranges
+      auto key_rank = [&](int64_t k) {
+        int r = 0;
+        while (r + 1 < nr && k > G.k1[r]) r++;
+        return pool[G.off[r] + (k - G.k0[r] - 1)];
+      };
+      // ---- exact first crossing ------------------------------------------
+      int64_t kst[PSOD_MAXDELTA];
+      uint64_t keyR = 0;
+      for (int di = 0; di < c.ndelta; di++) {
+        int32_t firstk = INT32_MAX;
+        for (int j = 0; j < G.nseg[di] && firstk == INT32_MAX; j++) {
+          const int64_t klo = max((int64_t)ldcg(&C[G.sg_s[di][j]]) + 1,
+                                  (int64_t)c.nmin_count);
+          const int64_t khi = ldcg(&C[G.sg_e[di][j] + 1]);
+          int32_t fk = INT32_MAX;
+          for (int64_t k = klo + threadIdx.x; k <= khi; k += GL_T) {
+            const double rk = (double)sqrtf(key_d2(key_rank(k)));
+            if (od_k(k, c.mpart, rk) <= c.delta[di]) {
+              fk = (int32_t)k;
+              break;
+            }
+          }
+          firstk = block_min_i32(fk, G);
+        }
+        kst[di] = firstk == INT32_MAX ? -1 : firstk;
+        if (kst[di] < 0) { ok = false; break; }
+        const uint64_t k1 = key_rank(kst[di] - 1);
+        const double r1 = ref_dist(pos, key_idx(k1), xcs, c.nx);
+        const double r2 = ref_dist(pos, key_idx(key_rank(kst[di])), xcs, c.nx);
+        finish_delta(c, di, kst[di], r1, r2, res);
+        if (di == 0) keyR = k1;
+      }
+      if (!ok) {
+        W = 2 * W;
+        __syncthreads();
+        continue;
+      }
+      const double R0 = res.radius[0];
+      if (c.rmax_frac * R0 > Wd) {
+        const int32_t need_w = (int32_t)ceil(c.rmax_frac * R0) + 1;
+        W = max(2 * W, need_w);
+        __syncthreads();
+        continue;
+      }
+      // ---- second-threshold bins (last k above delta2) ----------------------
+      // A bin may hold a rank k <= k0s with od_k > delta2 when its largest
+      // such rank could at the bin's inner edge (possible), and must at the
+      // outer edge (definite).  The answer lies in a possible bin between
+      // the last definite bin (or bin 0) and the last possible bin; those
+      // bins are collected as up to SEGCAP segments found downward, the
+      // lowest one extended to the floor.
+      const int64_t k0s = kst[0];
+      int ns2 = 0;
+      if (c.delta2 > 0.0) {
+        auto kmax2 = [&](int32_t b, int64_t *kmax) {
+          const int32_t cb = ldcg(&C[b]), ce = ldcg(&C[b + 1]);
+          *kmax = min((int64_t)ce, k0s);
+          return *kmax >= max((int64_t)cb + 1, (int64_t)2);
+        };
+        auto possible2 = [&](int32_t b) {
+          int64_t k;
+          return kmax2(b, &k) && rad3_for(k, c.mpart, c.delta2) > edge3_lo(b, nb, W3);
+        };
+        auto definite2 = [&](int32_t b) {
+          int64_t k;
+          return kmax2(b, &k) && rad3_for(k, c.mpart, c.delta2) >= edge3_hi(b, nb, W3);
+        };
+        // ranks above k0s do not count: no bin beyond the one of rank k0s
+        const int32_t bk0 = rbin(key_d2(key_rank(k0s)), scale, nb);
+        const int32_t b3 = block_last(0, bk0, possible2, G);
+        if (b3 >= 0) {
+          const int32_t b4 = block_last(0, b3, definite2, G);
+          const int32_t floor2 = b4 >= 0 ? b4 : 0;
+          int32_t x = b3;
+          while (x >= floor2) {
+            const int32_t e = block_last(floor2, x, possible2, G);
+            if (e < 0) break;
+            int32_t s0 = floor2;
+            if (ns2 < SEGCAP - 1) {
+              const int32_t q = block_last(
+                  floor2, e - 1,
+                  [&](int32_t b) { return nonempty(b) && !possible2(b); }, G);
+              if (q >= 0) s0 = q + 1;
+            }
+            // join the segment above when few keys lie between
+            const bool join = ns2 > 0 && ldcg(&C[x + 1]) - ldcg(&C[e + 1]) <= GAP_KEYS;
+            if (threadIdx.x == 0) {
+              if (join) {
+                G.rs[R2 + ns2 - 1] = s0;
+              } else {
+                G.rs[R2 + ns2] = s0;
+                G.re[R2 + ns2] = e;
+              }
+            }
+            if (!join) ns2++;
+            if (s0 <= floor2) break;
+            x = s0 - 1;
+          }
+        }
+        if (threadIdx.x == 0) {
+          int64_t o = G.bump;
+          for (int j = 0; j < ns2; j++) {
+            G.k0[R2 + j] = ldcg(&C[G.rs[R2 + j]]);
+            G.k1[R2 + j] = ldcg(&C[G.re[R2 + j] + 1]);
+            G.off[R2 + j] = o;
+            G.fill[R2 + j] = 0;
+            o += pool_slots(G.k1[R2 + j] - G.k0[R2 + j]);
+          }
+          G.bump = o;
+          G.need = o;
+        }
+      }
+      // profile radii
       for (int j = threadIdx.x; j < c.nbins; j += GL_T) {
-        int32_t s = 0;
-        for (int w = 0; w < GL_T / 64; w++) s += G.pcnt[w][j];
-        G.pcnt[0][j] = s;
+        G.prad[j] = prof_radius(c, R0, j);
+        for (int w = 0; w < GL_T / 64; w++) G.pcnt[w][j] = 0;
       }
       __syncthreads();
-      grow_finish(c, G, pool, pos, V, res, xcs, S, out);
-      step = GS_DONE;
+      if (G.need > pool_cap) { failed = true; break; }
+      // ---- pass 3: profile, centre of mass, boundary, band of delta2 -------
+      const double rlast = G.prad[c.nbins - 1];
+      double sp[3] = {0, 0, 0}, sv[3] = {0, 0, 0};
+      int64_t nbd = 0;
+      const int flagv = (c.flags & PSOD_F_VELOCITY) && vel;
+      // farthest particle this pass uses: profile, boundary band, centre of
+      // mass (keys up to keyR) and the second threshold's bins
+      float phi2;
+      {
+        const double rm = fmax(rlast, R0 + c.tol_abs);
+        float h2 = fmaxf((float)(rm * rm * (1.0 + 1e-4)) + 1e-6f, key_d2(keyR));
+        for (int j = 0; j < ns2; j++) {
+          float l2, u2;
+          bin_band(G.rs[R2 + j], G.re[R2 + j], nb, scale, W2, &l2, &u2);
+          h2 = fmaxf(h2, u2);
+        }
+        phi2 = fminf(h2, W2 * (1.0f + 1e-4f) + 1e-3f);
+      }
+      cube_walk(c, G, cell_start, rec, cr, W, -1.0f, phi2,
+                [&](float d2, int32_t idx, const float4 &r, int di, int dj,
+                    int dk, const int32_t *lc) {
+                  if (!(d2 < W2)) return;
+                  const double rr = (double)sqrtf(d2);
+                  if (rr < rlast) {
+                    int lo = 0, hi = c.nbins - 1;  // first j with rr < prad[j]
+                    while (lo < hi) {
+                      const int mid = (lo + hi) >> 1;
+                      if (rr < G.prad[mid]) hi = mid; else lo = mid + 1;
+                    }
+                    atomicAdd(&G.pcnt[threadIdx.x >> 6][lo], 1);
+                  }
+                  if (fabs(rr - R0) <= c.tol_abs) nbd++;
+                  if (d2key(d2, idx) <= keyR) {
+                    sp[0] += rel1d(r.x, lc[0], di, (double)cr.f[0]);
+                    sp[1] += rel1d(r.y, lc[1], dj, (double)cr.f[1]);
+                    sp[2] += rel1d(r.z, lc[2], dk, (double)cr.f[2]);
+                    if (flagv)
+                      for (int d = 0; d < 3; d++)
+                        sv[d] += (double)vel[3 * (int64_t)idx + d];
+                  }
+                  if (ns2 > 0) {
+                    const int32_t b = rbin(d2, scale, nb);
+                    for (int j = 0; j < ns2; j++)
+                      if (b >= G.rs[R2 + j] && b <= G.re[R2 + j]) {
+                        const int32_t s = atomicAdd(&G.fill[R2 + j], 1);
+                        pool[G.off[R2 + j] + s] = d2key(d2, idx);
+                      }
+                  }
+                });
+      nbd = block_sum_i64(nbd, G);
+      const int32_t nin = res.count[0];
+      const double xc[3] = {(double)cr.c[0] + (double)cr.f[0],
+                            (double)cr.c[1] + (double)cr.f[1],
+                            (double)cr.c[2] + (double)cr.f[2]};
+      for (int d = 0; d < 3; d++) {
+        const double s = block_sum_f64(sp[d], G);
+        const double v = block_sum_f64(sv[d], G);
+        double cp = (xc[d] + s / (double)nin) / (double)c.nx;
+        cp -= floor(cp);
+        res.com_pos[d] = cp;
+        res.com_vel[d] = v / (double)nin;
+      }
+      // cumulative profile
+      if (threadIdx.x == 0) {
+        int32_t s = 0;
+        for (int j = 0; j < c.nbins; j++) {
+          for (int w = 0; w < GL_T / 64; w++) s += G.pcnt[w][j];
+          out.profile[h * c.nbins + j] = s;
+        }
+      }
+      // ---- delta2 --------------------------------------------------------
+      if (c.delta2 > 0.0) {
+        int64_t best = 0;
+        uint64_t bkey = 0;
+        if (ns2 > 0) {
+          __threadfence_block();
+          __syncthreads();
+          // segments from the top down: the first one holding a rank above
+          // delta2 holds the last such rank
+          for (int j = 0; j < ns2 && best == 0; j++) {
+            const int32_t n2 = (int32_t)(G.k1[R2 + j] - G.k0[R2 + j]);
+            uint64_t *p2 = pool + G.off[R2 + j];
+            block_sort_pool(p2, n2, G);
+            const int64_t kb = G.k0[R2 + j];
+            int32_t bestk = -1;
+            for (int32_t t = threadIdx.x; t < n2; t += GL_T) {
+              const int64_t k = kb + t + 1;
+              if (k < 2 || k > k0s) continue;
+              const double rk = (double)sqrtf(key_d2(p2[t]));
+              if (od_k(k, c.mpart, rk) > c.delta2) bestk = max(bestk, (int32_t)k);
+            }
+            bestk = block_max_i32(bestk, G);
+            if (bestk > 0) {
+              best = bestk;
+              bkey = p2[bestk - kb - 1];
+            }
+          }
+        }
+        res.count2 = (int32_t)best;
+        if (best > 0) {
+          const double rb = ref_dist(pos, key_idx(bkey), xcs, c.nx);
+          res.overdens2 = rb > 1e-30 ? od_k(best, c.mpart, rb) : 0.0;
+        }
+      }
+      res.w = W;
+      res.ngather = (int32_t)ng;
+      res.nboundary = (int32_t)nbd;
+      bool acc = (double)res.count[0] * c.mpart >= c.mmin;
+      for (int d = 0; d < 3; d++) acc = acc && xc[d] > 0.0 && xc[d] <= c.nx;
+      res.status = PSOD_H_CLOSED | PSOD_H_LARGE | (acc ? PSOD_H_ACCEPTED : 0);
+      break;
     }
-    if (!deferred)
-      grow_close(c, counters, retry, needmax, G.need, V, res,
-                 step == GS_FAILED, out);
+    if (failed) res.status = PSOD_H_LARGE;  // band beyond the slice
+    if (threadIdx.x == 0 && !deferred) {
+      if (failed) {
+        const int32_t k = atomicAdd(&counters[4], 1);
+        if (retry) retry[k] = e;
+        if (needmax) atomicMax(needmax, (unsigned long long)G.need);
+      }
+      if (res.status & PSOD_H_UNCLOSED) atomicAdd(&counters[5], 1);
+      store_result(out, h, c.nbins, res);
+    }
     __syncthreads();
   }
 }
