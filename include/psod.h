@@ -79,7 +79,7 @@ int psod_sod_stats(psod_ctx *ctx, int64_t *out, int32_t n);
  *   small_wg_per_cu  one-wavefront halo workgroups per CU [8, at most 32]
  *   giant_min        particles in a growth cube from which the halo is grown
  *                    by the staged launches, each of its walks spread over
- *                    the whole GPU [4194304] */
+ *                    the whole GPU [1048576] */
 typedef struct {
   int32_t small_max, bound_min, dense_tile, small_wg_per_cu, giant_min;
 } psod_tunables;

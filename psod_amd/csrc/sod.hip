@@ -3034,7 +3034,7 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
 // again with a larger reach.  No launch waits on another workgroup: the
 // phases meet only at launch boundaries.
 // ---------------------------------------------------------------------------
-constexpr int32_t GIANT_CAP = 64;  // giant halos staged per pass
+constexpr int32_t GIANT_CAP = 256;  // giant halos staged per pass
 
 struct GiantState {
   GrowVars V;
@@ -3188,6 +3188,60 @@ __global__ void k_giant_layout(GiantState *__restrict__ S, int32_t ngiant,
   lay[ngiant] = off;
 }
 
+// Prefix of the chunks of the giants in phase `phase`: pre[g] = chunks of
+// the giants before g that are in the phase, pre[ngiant] = all of them.
+__global__ void k_giant_chunks(const GiantState *__restrict__ S,
+                               int32_t ngiant, int32_t phase,
+                               int32_t *__restrict__ pre) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  int32_t p = 0;
+  for (int32_t g = 0; g < ngiant; g++) {
+    pre[g] = p;
+    if (S[g].step == phase) p += S[g].nch;
+  }
+  pre[ngiant] = p;
+}
+
+// Exclusive prefix sums of the per-segment counts of each giant in
+// GS_COUNT (one workgroup per giant, 4096 segments per round).
+__global__ void __launch_bounds__(1024)
+k_giant_segscan(const GiantState *__restrict__ S,
+                const int32_t *__restrict__ segcnt,
+                int32_t *__restrict__ segpre) {
+  __shared__ int32_t wtot[16];
+  const GiantState &T = S[blockIdx.x];
+  if (T.step != GS_COUNT) return;  // uniform per workgroup
+  const int32_t *in = segcnt + T.segoff;
+  int32_t *out = segpre + T.segoff;
+  const int32_t n = T.cseg;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  int32_t carry = 0;
+  for (int32_t base = 0; base < n; base += 4096) {
+    const int32_t i0 = base + 4 * (int32_t)threadIdx.x;
+    int32_t v[4], sum = 0;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      v[u] = i0 + u < n ? in[i0 + u] : 0;
+      sum += v[u];
+    }
+    const int32_t inc = wave_incl_scan(sum);
+    if (l == 63) wtot[w] = inc;
+    __syncthreads();
+    int32_t run = carry + inc - sum, all = 0;
+    for (int i = 0; i < 16; i++) {
+      if (i < w) run += wtot[i];
+      all += wtot[i];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      if (i0 + u < n) out[i0 + u] = run;
+      run += v[u];
+    }
+    carry += all;
+    __syncthreads();
+  }
+}
+
 // chunks of CH_PART particles of a cube of ncube particles
 __device__ inline int32_t part_chunks(int64_t ncube) {
   return (int32_t)max((int64_t)1, (ncube + CH_PART - 1) / CH_PART);
@@ -3256,23 +3310,27 @@ k_giant_walk(SodCfg c, int32_t phase, const int32_t *__restrict__ cell_start,
              GiantState *__restrict__ S, int32_t ngiant,
              int32_t *__restrict__ hist_all, uint64_t *__restrict__ pool_all,
              int64_t pool_cap, int32_t *__restrict__ segcnt,
-             const int32_t *__restrict__ segpre, int32_t *__restrict__ wq) {
+             const int32_t *__restrict__ segpre,
+             const int32_t *__restrict__ chpre, int32_t *__restrict__ wq) {
   __shared__ GrowShared G;
   __shared__ int32_t lh[HIST_LDS];
   __shared__ int32_t s_g, s_k, s_q0, s_q1;
+  const int32_t total = chpre[ngiant];
   for (;;) {
-    if (threadIdx.x == 0) {
-      int32_t k = atomicAdd(wq, 1), g = -1;
-      for (int32_t i = 0; i < ngiant; i++) {
-        if (S[i].step != phase) continue;
-        if (k < S[i].nch) {
-          g = i;
-          break;
-        }
-        k -= S[i].nch;
+    // item it: giant g with chpre[g] <= it < chpre[g + 1] (the giants out of
+    // the phase hold no chunks), its chunk it - chpre[g]
+    if (threadIdx.x < 64) {
+      const int32_t it = __builtin_amdgcn_readfirstlane(
+          threadIdx.x == 0 ? atomicAdd(wq, 1) : 0);
+      int32_t g = -1, k = 0;
+      if (it < total) {
+        g = wave_lower_bound(chpre + 1, ngiant, (int64_t)it + 1);
+        k = it - chpre[g];
       }
-      s_g = g;
-      s_k = k;
+      if (threadIdx.x == 0) {
+        s_g = g;
+        s_k = k;
+      }
     }
     __syncthreads();
     const int32_t g = s_g, k = s_k;
@@ -3683,6 +3741,7 @@ struct SodScratch {
   unsigned long long *hbest;
   int32_t *giants;     // (e, W) of the giant halos left to the staged growth
   int64_t *glay;       // their per-segment count layout (k_giant_layout)
+  int32_t *gchunk;     // prefix of their chunks in a walk (k_giant_chunks)
   GiantState *gstate;  // their states, GIANT_CAP entries
 };
 
@@ -3695,12 +3754,18 @@ static int sod_scratch(psod_ctx *ctx, int64_t nhalo, SodScratch *S) {
   const int32_t ncap = (int32_t)nhalo;
   int32_t *ntask = (int32_t *)psod_scratch(
       ctx, S_MISC, (size_t)(5 * ncap + 128) * 4 + (size_t)ncap * 16 + 512);
+  // giant list (2 * GIANT_CAP), layout (GIANT_CAP + 1), chunk prefix
+  // (GIANT_CAP + 1), states (GIANT_CAP), each 256-byte aligned
+  constexpr size_t G_LIST = 0, G_LAY = 2 * GIANT_CAP * 4,
+                   G_CH = G_LAY + ((GIANT_CAP + 1) * 8 + 255) / 256 * 256,
+                   G_ST = G_CH + ((GIANT_CAP + 1) * 4 + 255) / 256 * 256;
   char *gb = (char *)psod_scratch(
-      ctx, S_GIANT, 1536 + (size_t)GIANT_CAP * sizeof(GiantState));
+      ctx, S_GIANT, G_ST + (size_t)GIANT_CAP * sizeof(GiantState));
   if (!S->plan || !lists || !ntask || !gb) return PSOD_E_NOMEM;
-  S->giants = (int32_t *)gb;            // 2 * GIANT_CAP entries
-  S->glay = (int64_t *)(gb + 512);      // GIANT_CAP + 1 entries
-  S->gstate = (GiantState *)(gb + 1536);
+  S->giants = (int32_t *)(gb + G_LIST);
+  S->glay = (int64_t *)(gb + G_LAY);
+  S->gchunk = (int32_t *)(gb + G_CH);
+  S->gstate = (GiantState *)(gb + G_ST);
   S->small = lists;
   S->large = lists + nhalo;
   S->counters = lists + 2 * nhalo;
@@ -3932,9 +3997,12 @@ static int grow_giants(psod_ctx *ctx, const SodCfg &c,
   PSOD_LAUNCH_CHECK();
   int32_t *segcnt = nullptr, *segpre = nullptr;
   auto walk = [&](int32_t phase, int32_t *q) -> int {
+    k_giant_chunks<<<1, 64, 0, st>>>(gs, ngiant, phase, S.gchunk);
+    PSOD_LAUNCH_CHECK();
     k_giant_walk<<<nwalk, GL_T, 0, st>>>(c, phase, cat->cell_start, rec,
                                          cat->vel, gs, ngiant, hist, pool,
-                                         pool_cap, segcnt, segpre, q);
+                                         pool_cap, segcnt, segpre, S.gchunk,
+                                         q);
     PSOD_LAUNCH_CHECK();
     return PSOD_OK;
   };
@@ -3960,10 +4028,8 @@ static int grow_giants(psod_ctx *ctx, const SodCfg &c,
     if (!segcnt) return PSOD_E_NOMEM;
     segpre = segcnt + lay[ngiant] + 32;
     PSOD_TRY(walk(GS_COUNT, wq + 0));
-    for (int32_t g = 0, off = 0; g < ngiant; off += (int32_t)lay[g], g++)
-      if (lay[g] > 0)
-        PSOD_TRY(psod_scan_i32(ctx, segcnt + off, segpre + off, lay[g],
-                               nullptr));
+    k_giant_segscan<<<ngiant, 1024, 0, st>>>(gs, segcnt, segpre);
+    PSOD_LAUNCH_CHECK();
     k_giant_size<<<ngiant, GL_T, 0, st>>>(c, gs, hist, wq);
     PSOD_LAUNCH_CHECK();
     PSOD_TRY(walk(GS_HIST, wq + 1));

@@ -25,7 +25,7 @@ def test_tunables_do_not_change_results(ctx):
     nx = p.nx
     base = ctx.tunables(nx)
     assert base == dict(small_max=1024, bound_min=2048, dense_tile=16384,
-                        small_wg_per_cu=8, giant_min=1 << 22)
+                        small_wg_per_cu=8, giant_min=1 << 20)
     runs = []
     for t in (dict(), dict(small_max=256, bound_min=64, dense_tile=512,
                            small_wg_per_cu=2, giant_min=2000),
@@ -104,7 +104,7 @@ def test_autotune_file_round_trip(ctx, tmp_path):
 def test_staged_growth_of_giant_halos(ctx):
     """A 128^3 catalogue with a 1.5e6-particle central clump: halos grown
     by the staged launches (giant_min at or below their cubes; at 4096 more
-    halos qualify than the 64 staged per pass, so the rest stay on one
+    halos qualify than the 256 staged per pass, so the rest stay on one
     workgroup each) give the results of one workgroup per halo (giant_min
     above every cube)."""
     import psod_amd as P
