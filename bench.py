@@ -321,7 +321,10 @@ def main():
     torch.cuda.synchronize()
     eager_ms = (time.perf_counter() - t0) / args.steps * 1e3
     cs = ctx.sod_stats()  # execution paths of the last eager step
-    st = {n: float(np.mean([a.elapsed_time(b) for a, b in ev[n]]))
+    # per-stage median over the timed steps: a step that waits on a host-side
+    # stall (seen now and then in later processes on a box, inside a stage's
+    # events) does not move it
+    st = {n: float(np.median([a.elapsed_time(b) for a, b in ev[n]]))
           for n in names}
     log(f"eager step {eager_ms:.2f} ms  stages(ms) " +
         " ".join(f"{k}={v:.3f}" for k, v in st.items()))

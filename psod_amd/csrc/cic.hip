@@ -322,6 +322,18 @@ k_cic_interp(const int32_t *__restrict__ cell_start,
   const int32_t blk = blockIdx.x;
   const int bx = blk % nb, by = (blk / nb) % nb, bz = blk / (nb * nb);
   const int32_t gx0 = bx * 8 * R, gy0 = by * 8 * R, gz0 = bz * 8 * R;
+  const bool seeds = seed_key && lab_sorted;
+  // The block's record range first; then the first round's record and label
+  // are requested before the nodes are staged, and each round requests the
+  // next round's before its own are used, so that the loads of a block are
+  // in flight together rather than one after the other.
+  const int32_t s0 = cell_start[blk * BS3], s1 = cell_start[blk * BS3 + BS3];
+  float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
+  int32_t h = -1;
+  if (s0 + (int32_t)threadIdx.x < s1) {
+    r = rec[s0 + threadIdx.x];
+    if (seeds) h = lab_sorted[s0 + threadIdx.x];
+  }
   for (int i = threadIdx.x; i < NN * NN * NN; i += 256) {
     const int ax = i % NN, ay = (i / NN) % NN, az = i / (NN * NN);
     const int32_t gx = (gx0 + ax + shift) & (ng - 1);
@@ -330,16 +342,20 @@ k_cic_interp(const int32_t *__restrict__ cell_start,
     g[i] = grid[((int64_t)gz * ng + gy) * ng + gx];
   }
   __syncthreads();
-  const int32_t s0 = cell_start[blk * BS3], s1 = cell_start[blk * BS3 + BS3];
   // uniform trip count so that every lane of a wave takes part in the
   // per-label reduction of the seed keys
   for (int32_t base = s0; base < s1; base += 256) {
     const int32_t k = base + threadIdx.x;
     const bool ok = k < s1;
+    float4 rn = make_float4(0.f, 0.f, 0.f, 0.f);
+    int32_t hn = -1;
+    if (k + 256 < s1) {
+      rn = rec[k + 256];
+      if (seeds) hn = lab_sorted[k + 256];
+    }
     float rf = 0.0f;
     int32_t idx = 0;
     if (ok) {
-      const float4 r = rec[k];
       const float qx = (float)R * r.x, qy = (float)R * r.y, qz = (float)R * r.z;
       const float fx0 = floorf(qx), fy0 = floorf(qy), fz0 = floorf(qz);
       const int ix = (int)fx0, iy = (int)fy0, iz = (int)fz0;
@@ -360,9 +376,8 @@ k_cic_interp(const int32_t *__restrict__ cell_start,
       if (rho_sorted) rho_sorted[k] = rf;
       if (rho) rho[idx] = rf;
     }
-    if (seed_key && lab_sorted) {
-      const int32_t h = ok ? lab_sorted[k] : -1;
-      const bool mem = h >= 0 && h < nhalo;
+    if (seeds) {
+      const bool mem = ok && h >= 0 && h < nhalo;
       const unsigned long long key =
           mem ? (((unsigned long long)__float_as_uint(rf) << 32) |
                  (unsigned long long)(0xFFFFFFFFu - (uint32_t)idx))
@@ -383,6 +398,8 @@ k_cic_interp(const int32_t *__restrict__ cell_start,
         todo &= ~__ballot(mine);
       }
     }
+    r = rn;
+    h = hn;
   }
 }
 
