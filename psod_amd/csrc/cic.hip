@@ -83,6 +83,7 @@ __device__ inline void tile_seg(int s, int *bi, int *bj, int *bk, int *loc,
   *loc = 511; *len = 1;
 }
 
+constexpr int DEP_T = 512;  // threads of a deposit workgroup (one tile)
 constexpr int DEP_K = 4;  // consecutive particles per lane
 constexpr int32_t DEP_CAPD = 8192;  // dense tiles dispatched first, at most
 
@@ -134,7 +135,7 @@ __device__ inline int32_t tile_of_block(int32_t b, int32_t ntile, int32_t capd,
 // from TB^2 contiguous runs of TB blocks; halo particles from the ranges of
 // tile_seg(), found by binary search on their prefix counts.
 template <int R, int TB>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(DEP_T)
 k_cic_deposit(const int32_t *__restrict__ cell_start,
               const float4 *__restrict__ rec, int32_t nx, int32_t ng,
               int32_t shift, double norm, int32_t capd,
@@ -149,7 +150,7 @@ k_cic_deposit(const int32_t *__restrict__ cell_start,
   __shared__ int8_t hoff[NS][3];
   __shared__ int32_t ostart[TB * TB], opre[TB * TB + 1];
   __shared__ int32_t obnd[TB * TB][TB];  // own-list position of each block
-  __shared__ int32_t wtot[4];
+  __shared__ int32_t wtot[DEP_T / 64];
   const int32_t nb = nx >> 3;
   const int32_t nt = nb / TB;  // tiles per axis
   const int32_t tile = tile_of_block(blockIdx.x, nt * nt * nt, capd, dcount,
@@ -158,7 +159,7 @@ k_cic_deposit(const int32_t *__restrict__ cell_start,
   const int tx = tile % nt, ty = (tile / nt) % nt, tz = tile / (nt * nt);
   const int bx0 = tx * TB, by0 = ty * TB, bz0 = tz * TB;
   const int l = threadIdx.x & 63;
-  for (int i = threadIdx.x; i < NN * NN * NN; i += 256) acc[i] = 0ull;
+  for (int i = threadIdx.x; i < NN * NN * NN; i += DEP_T) acc[i] = 0ull;
   // own runs: blocks (bx0..bx0+TB-1, by0+j, bz0+k) are consecutive keys
   if (threadIdx.x < TB * TB) {
     const int j = threadIdx.x % TB, k = threadIdx.x / TB;
@@ -168,9 +169,9 @@ k_cic_deposit(const int32_t *__restrict__ cell_start,
       obnd[threadIdx.x][q] = cell_start[(b0 + q) * BS3];
     opre[threadIdx.x + 1] = cell_start[(b0 + TB) * BS3];
   }
-  // halo ranges and their prefix counts (blocks of 256 ranges)
+  // halo ranges and their prefix counts (blocks of DEP_T ranges)
   int32_t carry = 0;
-  for (int base = 0; base < NS; base += 256) {
+  for (int base = 0; base < NS; base += DEP_T) {
     const int s = base + threadIdx.x;
     int32_t cnt = 0;
     if (s < NS) {
@@ -194,7 +195,7 @@ k_cic_deposit(const int32_t *__restrict__ cell_start,
     int32_t off = carry;
     for (int i = 0; i < w; i++) off += wtot[i];
     if (s < NS) hpre[s] = off + inc - cnt;
-    carry += wtot[0] + wtot[1] + wtot[2] + wtot[3];
+    for (int i = 0; i < DEP_T / 64; i++) carry += wtot[i];
     __syncthreads();
   }
   if (threadIdx.x == 0) {
@@ -230,7 +231,7 @@ k_cic_deposit(const int32_t *__restrict__ cell_start,
   // All K records of a lane are requested before the first is used, so that
   // a wave keeps K loads per lane in flight.
   const int wv = threadIdx.x >> 6;
-  for (int32_t base = 0; base < ntot; base += 256 * DEP_K) {
+  for (int32_t base = 0; base < ntot; base += DEP_T * DEP_K) {
     float4 rv[DEP_K];
     int32_t ob[DEP_K];  // block offset of the record, packed; 0 = none
 #pragma unroll
@@ -297,7 +298,7 @@ k_cic_deposit(const int32_t *__restrict__ cell_start,
   flush();
   __syncthreads();
   const int32_t gx0 = bx0 * 8 * R, gy0 = by0 * 8 * R, gz0 = bz0 * 8 * R;
-  for (int i = threadIdx.x; i < NN * NN * NN; i += 256) {
+  for (int i = threadIdx.x; i < NN * NN * NN; i += DEP_T) {
     const int ax = i % NN, ay = (i / NN) % NN, az = i / (NN * NN);
     const int32_t gx = (gx0 + ax + shift) & (ng - 1);
     const int32_t gy = (gy0 + ay + shift) & (ng - 1);
@@ -447,15 +448,15 @@ static int deposit_queue(psod_ctx *ctx, int64_t npart, int32_t nx,
     PSOD_LAUNCH_CHECK();
   }
   if (ng == nx && TB == 2)
-    k_cic_deposit<1, 2><<<nblk, 256, 0, st>>>(
+    k_cic_deposit<1, 2><<<nblk, DEP_T, 0, st>>>(
         d_cell_start, (const float4 *)d_rec, nx, ng, shift, norm, capd, dcount,
         dlist, listed, d_grid);
   else if (ng == nx)
-    k_cic_deposit<1, 1><<<nblk, 256, 0, st>>>(
+    k_cic_deposit<1, 1><<<nblk, DEP_T, 0, st>>>(
         d_cell_start, (const float4 *)d_rec, nx, ng, shift, norm, capd, dcount,
         dlist, listed, d_grid);
   else
-    k_cic_deposit<2, 1><<<nblk, 256, 0, st>>>(
+    k_cic_deposit<2, 1><<<nblk, DEP_T, 0, st>>>(
         d_cell_start, (const float4 *)d_rec, nx, ng, shift, norm, capd, dcount,
         dlist, listed, d_grid);
   PSOD_LAUNCH_CHECK();
