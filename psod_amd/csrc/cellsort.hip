@@ -306,22 +306,56 @@ k_block_sort(const int32_t *__restrict__ bstart, int64_t nblk,
   __shared__ int32_t cnt[512];
   __shared__ float4 sr[BS_CAP];
   __shared__ int32_t sl[BS_CAP];
-  for (int64_t b = blockIdx.x; b < nblk; b += gridDim.x) {
-    const int32_t a = bstart[b], n = bstart[b + 1] - a;
+  // A staged block's records are requested one block ahead (while the
+  // block before is placed), and block ranges two ahead, so that the loads
+  // of a block are not waited for one after the other.
+  auto load = [&](int32_t a, int32_t n, float4 *rr, int32_t *ll) {
+    if (n > BS_CAP) return;
+#pragma unroll
+    for (int k = 0; k < BS_U; k++) {
+      const int32_t t = threadIdx.x + 256 * k;
+      if (t < n) {
+        const float4 v = stage[a + t];
+        rr[k].x = v.x;
+        rr[k].y = v.y;
+        rr[k].z = v.z;
+        rr[k].w = v.w;
+        ll[k] = stage_lab ? stage_lab[a + t] : 0;
+      }
+    }
+  };
+  const int64_t G = gridDim.x;
+  int64_t b = blockIdx.x;
+  int32_t a = 0, n = 0, an = 0, nn = 0;
+  if (b < nblk) {
+    a = bstart[b];
+    n = bstart[b + 1] - a;
+  }
+  if (b + G < nblk) {
+    an = bstart[b + G];
+    nn = bstart[b + G + 1] - an;
+  }
+  float4 rr[BS_U];
+  int32_t ll[BS_U];
+  load(a, n, rr, ll);
+  for (; b < nblk; b += G) {
+    float4 rn[BS_U];
+    int32_t ln[BS_U];
+    load(an, nn, rn, ln);
+    int32_t a2 = 0, n2 = 0;
+    if (b + 2 * G < nblk) {
+      a2 = bstart[b + 2 * G];
+      n2 = bstart[b + 2 * G + 1] - a2;
+    }
     for (int i = threadIdx.x; i < 512; i += 256) cnt[i] = 0;
     __syncthreads();
     const bool staged = n <= BS_CAP;
-    float4 rr[BS_U];
-    int32_t ll[BS_U], dd[BS_U];
+    int32_t dd[BS_U];
     if (staged) {
 #pragma unroll
       for (int k = 0; k < BS_U; k++) {
         const int32_t t = threadIdx.x + 256 * k;
-        if (t < n) {
-          rr[k] = stage[a + t];
-          ll[k] = stage_lab ? stage_lab[a + t] : 0;
-          dd[k] = atomicAdd(&cnt[local_cell(rr[k])], 1);
-        }
+        if (t < n) dd[k] = atomicAdd(&cnt[local_cell(rr[k])], 1);
       }
     } else {
       for (int32_t t = threadIdx.x; t < n; t += 256)
@@ -391,6 +425,15 @@ k_block_sort(const int32_t *__restrict__ bstart, int64_t nblk,
       }
     }
     __syncthreads();
+    a = an;
+    n = nn;
+    an = a2;
+    nn = n2;
+#pragma unroll
+    for (int k = 0; k < BS_U; k++) {
+      rr[k] = rn[k];
+      ll[k] = ln[k];
+    }
   }
 }
 

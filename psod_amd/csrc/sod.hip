@@ -3558,14 +3558,18 @@ __global__ void k_large_keys(SodCfg c, const HaloPlan *__restrict__ plan,
       lo[d] = floor8(pl.cs[d] - W);
       hi[d] = min(floor8(pl.cs[d] + W), lo[d] + nb - 1);  // at most nx cells
     }
+    // every s-th block per axis (all of them up to 16 per axis), scaled by
+    // s^3: the key only orders the halos
+    const int32_t s = (hi[0] - lo[0] + 16) / 16;
     cost = 0;
-    for (int32_t bz = lo[2]; bz <= hi[2]; bz++)
-      for (int32_t by = lo[1]; by <= hi[1]; by++)
-        for (int32_t bx = lo[0]; bx <= hi[0]; bx++) {
+    for (int32_t bz = lo[2]; bz <= hi[2]; bz += s)
+      for (int32_t by = lo[1]; by <= hi[1]; by += s)
+        for (int32_t bx = lo[0]; bx <= hi[0]; bx += s) {
           const int64_t b = ((int64_t)wrapi(bz, nb) * nb + wrapi(by, nb)) * nb +
                             wrapi(bx, nb);
           cost += cell_start[(b + 1) * BS3] - cell_start[b * BS3];
         }
+    cost *= (int64_t)s * s * s;
     // A cube already holding more particles than a sphere of radius
     // W / rmax_frac at the first threshold would is likely to be outgrown
     // and walked again at twice the reach (about 4x the particles): such a
