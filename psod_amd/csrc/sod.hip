@@ -750,6 +750,9 @@ k_centre_bound(SodCfg c, const int32_t *__restrict__ cell_start,
             w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int32_t nt = *ntask_total;
   const float cut = c.rs2f * 1.0001f + 1e-6f;  // conservative chunk culling
+  // held in a vector register: the term's fused multiply-add takes it
+  // without a move per term (71.8 -> 67.9 ms at 1024^3 with the maxima below)
+  const float eps2v = __shfl(c.eps2f, 0, 64);
   for (;;) {
     const int32_t t0 = wave_next(wq) * 2;
     if (t0 >= nt) break;
@@ -799,15 +802,17 @@ k_centre_bound(SodCfg c, const int32_t *__restrict__ cell_start,
         g1y = fmaxf(g1y, __shfl_xor(g1y, o, 64));
         g1z = fmaxf(g1z, __shfl_xor(g1z, o, 64));
       }
+      // minima in scalar registers, maxima in vector registers: the clamp
+      // (v_med3) takes one scalar operand, so neither needs a move per term
       float B[4][6];
 #pragma unroll
       for (int g = 0; g < 4; g++) {
         B[g][0] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g0x), 16 * g));
         B[g][1] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g0y), 16 * g));
         B[g][2] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g0z), 16 * g));
-        B[g][3] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g1x), 16 * g));
-        B[g][4] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g1y), 16 * g));
-        B[g][5] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g1z), 16 * g));
+        B[g][3] = __shfl(g1x, 16 * g, 64);
+        B[g][4] = __shfl(g1y, 16 * g, 64);
+        B[g][5] = __shfl(g1z, 16 * g, 64);
       }
       const int ngr = (tk.w + 7) >> 3;  // groups holding candidates
       const int nbx = (tk.w + 15) >> 4;  // boxes holding candidates
@@ -884,7 +889,7 @@ k_centre_bound(SodCfg c, const int32_t *__restrict__ cell_start,
               const float gy = vy - __builtin_amdgcn_fmed3f(vy, B[g][1], B[g][4]);
               const float gz = vz - __builtin_amdgcn_fmed3f(vz, B[g][2], B[g][5]);
               const float g2 = fmaf(gx, gx, fmaf(gy, gy, gz * gz));
-              ubg[g] += g2 <= cut ? sterm_bound(g2, c.eps2f) : 0u;
+              ubg[g] += g2 <= cut ? sterm_bound(g2, eps2v) : 0u;
             }
           }
         }
@@ -921,13 +926,154 @@ k_centre_bound(SodCfg c, const int32_t *__restrict__ cell_start,
   }
 }
 
+// A pilot task: its candidates (one per lane), their box, and the record
+// ranges of the 27 cells around the candidates' cell (lanes 0..26; a cell
+// beyond reach of the box is empty).
+struct PilotSetup {
+  float px, py, pz;
+  int32_t pidx;
+  bool have;
+  int ci, cj, ck;
+  float cb0[3], cb1[3];
+  int32_t qa, qb;
+};
+
+__device__ __forceinline__ void pilot_setup(const SodCfg &c,
+                                            const int32_t *cell_start,
+                                            const float4 *rec,
+                                            const HaloPlan &pl, const int4 &tk,
+                                            int l, float cut, PilotSetup &P) {
+  walk_off(tk.y, 1, &P.ci, &P.cj, &P.ck);
+  int32_t lc[3];
+  key_at(c.nx, pl.cs, P.ci, P.cj, P.ck, lc);
+  P.px = P.py = P.pz = 0.f;
+  P.pidx = 0;
+  P.have = l < tk.w;
+  if (P.have) {
+    const float4 r = rec[tk.z + l];
+    P.px = rel1(r.x, lc[0], P.ci, 0.0f);
+    P.py = rel1(r.y, lc[1], P.cj, 0.0f);
+    P.pz = rel1(r.z, lc[2], P.ck, 0.0f);
+    P.pidx = __float_as_int(r.w);
+  }
+  P.cb0[0] = wave_minf(P.have ? P.px : INFINITY);
+  P.cb0[1] = wave_minf(P.have ? P.py : INFINITY);
+  P.cb0[2] = wave_minf(P.have ? P.pz : INFINITY);
+  P.cb1[0] = wave_maxf(P.have ? P.px : -INFINITY);
+  P.cb1[1] = wave_maxf(P.have ? P.py : -INFINITY);
+  P.cb1[2] = wave_maxf(P.have ? P.pz : -INFINITY);
+  P.qa = P.qb = 0;
+  if (l < 27) {
+    int ei, ej, ek;
+    walk_off(l, 1, &ei, &ej, &ek);
+    const int di = P.ci + ei, dj = P.cj + ej, dk = P.ck + ek;
+    const float nb0[3] = {(float)di, (float)dj, (float)dk};
+    const float nb1[3] = {(float)di + 1.f, (float)dj + 1.f, (float)dk + 1.f};
+    if (box_gap2(P.cb0, P.cb1, nb0, nb1) <= cut) {
+      int32_t ln[3];
+      const int32_t key = key_at(c.nx, pl.cs, di, dj, dk, ln);
+      P.qa = cell_start[key];
+      P.qb = cell_start[key + 1];
+    }
+  }
+}
+
+// Exact sums of a pilot task's candidates (this lane's) over at most
+// `budget` chunks of 64 records of the cells in reach, starting with record
+// s of cell q and going on through the cells of `rest` (cells in walk_off
+// order).  The next chunk is requested before the current one is used; a
+// chunk whose box lies beyond reach of the candidates' box is skipped.
+// Adds the neighbours tested per candidate to npair.  nb is the wave's LDS
+// slot of 64 entries.
+__device__ __forceinline__ uint64_t pilot_walk(const SodCfg &c,
+                                               const float4 *rec,
+                                               const HaloPlan &pl,
+                                               const PilotSetup &P, float4 *nb,
+                                               int l, float cut, uint64_t rest,
+                                               int q, int32_t s, int32_t budget,
+                                               uint64_t &npair) {
+  uint64_t acc = 0;
+  float4 rn = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (q < 27 && s + l < __builtin_amdgcn_readlane(P.qb, q)) rn = rec[s + l];
+  while (q < 27 && budget > 0) {
+    budget--;
+    const int32_t b = __builtin_amdgcn_readlane(P.qb, q);
+    const int32_t m = min(64, b - s);
+    const float4 r = rn;
+    const int qc = q;
+    int32_t sn = s + 64;
+    if (sn >= b) {
+      rest &= ~(1ull << q);
+      q = rest ? __ffsll((unsigned long long)rest) - 1 : 27;
+      sn = q < 27 ? __builtin_amdgcn_readlane(P.qa, q) : 0;
+    }
+    if (q < 27 && budget > 0 && sn + l < __builtin_amdgcn_readlane(P.qb, q))
+      rn = rec[sn + l];
+    s = sn;
+    int ei, ej, ek;
+    walk_off(qc, 1, &ei, &ej, &ek);
+    const int di = P.ci + ei, dj = P.cj + ej, dk = P.ck + ek;
+    // block-local coordinates of the walked cell (8 divides nx)
+    const int32_t lx = (pl.cs[0] + di) & 7, ly = (pl.cs[1] + dj) & 7,
+                  lz = (pl.cs[2] + dk) & 7;
+    float vx = 0.f, vy = 0.f, vz = 0.f;
+    if (l < m) {
+      vx = rel1(r.x, lx, di, 0.0f);
+      vy = rel1(r.y, ly, dj, 0.0f);
+      vz = rel1(r.z, lz, dk, 0.0f);
+    }
+    if (m >= 16) {
+      float nb0[3], nb1[3];
+      nb0[0] = wave_minf(l < m ? vx : INFINITY);
+      nb0[1] = wave_minf(l < m ? vy : INFINITY);
+      nb0[2] = wave_minf(l < m ? vz : INFINITY);
+      nb1[0] = wave_maxf(l < m ? vx : -INFINITY);
+      nb1[1] = wave_maxf(l < m ? vy : -INFINITY);
+      nb1[2] = wave_maxf(l < m ? vz : -INFINITY);
+      if (box_gap2(P.cb0, P.cb1, nb0, nb1) > cut) continue;
+    }
+    uint32_t part = 0;  // <= 64 terms of <= 2^24
+    // the chunk goes through the wave's LDS slot and is read back with
+    // wave-uniform addresses (broadcast reads, no lane moves)
+    nb[l] = make_float4(vx, vy, vz, 0.0f);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // m is uniform: a scalar trip count, and a select instead of a branch
+    const int32_t mu = __builtin_amdgcn_readfirstlane(m);
+#pragma unroll 4
+    for (int32_t u = 0; u < mu; u++) {
+      const float4 qv = nb[u];
+      const float dx = qv.x - P.px, dy = qv.y - P.py, dz = qv.z - P.pz;
+      const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+      const uint32_t term = sterm_fix(d2, c.eps2f);
+      part += d2 <= c.rs2f ? term : 0u;
+    }
+    acc += part;
+    npair += (uint64_t)m;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  return acc;
+}
+
+// Pilot tasks with more than pilot_big records in reach (2^15 unless
+// PSOD_PILOT_BIG says otherwise) are split into PILOT_PIECES pieces of their
+// chunks, each summed by its own wave into per-candidate totals (integers:
+// the grouping does not change S).  One wave summing a 5e7-particle halo's
+// densest task alone took 41 ms.
+constexpr int32_t PILOT_PIECES = 64;
+constexpr int32_t PILOT_BIGCAP = 1024;  // split tasks per pass; more go whole
+
 // Pilots, one wave per task: the 64 candidates of a task (one per lane)
 // against the 27 cells around their cell (every neighbour within 1 cell lies
 // there).  Evaluates exactly the tasks holding their halo's largest bound
 // (every task of a halo that skipped the bound) and folds their S into
 // hbest.  Dense cells are in Morton order of their 1/8-cell sub-cells, so
-// candidate groups and neighbour chunks are compact; a chunk whose box lies
-// beyond the reach of the candidates' box is skipped.
+// candidate groups and neighbour chunks are compact.  A task with more than
+// pilot_big records in reach is appended to big[] (count in *nbig, its
+// per-candidate totals in pacc cleared) and left to k_centre_pilot_big.
 __global__ void __launch_bounds__(256)
 k_centre_pilot(SodCfg c, const int32_t *__restrict__ cell_start,
                const float4 *__restrict__ rec, const HaloPlan *__restrict__ plan,
@@ -938,6 +1084,9 @@ k_centre_pilot(SodCfg c, const int32_t *__restrict__ cell_start,
                const unsigned long long *__restrict__ hpil,
                const unsigned long long *__restrict__ ubt,
                unsigned long long *__restrict__ cstat,
+               int32_t pilot_big, int32_t *__restrict__ big,
+               int32_t *__restrict__ nbig,
+               unsigned long long *__restrict__ pacc,
                int32_t *__restrict__ wq) {
   __shared__ float4 nbuf[4][64];  // one neighbour chunk per wave
   const int l = threadIdx.x & 63,
@@ -954,113 +1103,28 @@ k_centre_pilot(SodCfg c, const int32_t *__restrict__ cell_start,
       const int4 tk = tasks[t];
       if (ubt[t] != hpil[tk.x]) continue;
       const HaloPlan pl = plan[large[tk.x]];
-      int ci, cj, ck;
-      walk_off(tk.y, 1, &ci, &cj, &ck);
-      int32_t lc[3];
-      key_at(c.nx, pl.cs, ci, cj, ck, lc);
-      float px = 0.f, py = 0.f, pz = 0.f;
-      int32_t pidx = 0;
-      const bool have = l < tk.w;
-      if (have) {
-        const float4 r = rec[tk.z + l];
-        px = rel1(r.x, lc[0], ci, 0.0f);
-        py = rel1(r.y, lc[1], cj, 0.0f);
-        pz = rel1(r.z, lc[2], ck, 0.0f);
-        pidx = __float_as_int(r.w);
+      PilotSetup P;
+      pilot_setup(c, cell_start, rec, pl, tk, l, cut, P);
+      if (wave_sum(P.qb - P.qa) > pilot_big) {
+        int32_t k = 0;
+        if (l == 0) k = atomicAdd(nbig, 1);
+        k = __builtin_amdgcn_readfirstlane(k);
+        if (k < PILOT_BIGCAP) {
+          if (l == 0) big[k] = t;
+          pacc[(int64_t)k * 64 + l] = 0;
+          continue;
+        }
       }
-      float cb0[3], cb1[3];
-      cb0[0] = wave_minf(have ? px : INFINITY);
-      cb0[1] = wave_minf(have ? py : INFINITY);
-      cb0[2] = wave_minf(have ? pz : INFINITY);
-      cb1[0] = wave_maxf(have ? px : -INFINITY);
-      cb1[1] = wave_maxf(have ? py : -INFINITY);
-      cb1[2] = wave_maxf(have ? pz : -INFINITY);
-      uint64_t acc = 0;
+      uint64_t rest = __ballot(l < 27 && P.qa < P.qb);  // cells with records
+      const int q = rest ? __ffsll((unsigned long long)rest) - 1 : 27;
+      const int32_t s = q < 27 ? __builtin_amdgcn_readlane(P.qa, q) : 0;
       uint64_t npair = 0;  // neighbours tested per candidate
-      // record ranges of the 27 cells, one per lane, all loaded at once; a cell
-      // beyond reach of the candidates' box is empty
-      int32_t qa = 0, qb = 0;
-      if (l < 27) {
-        int ei, ej, ek;
-        walk_off(l, 1, &ei, &ej, &ek);
-        const int di = ci + ei, dj = cj + ej, dk = ck + ek;
-        const float nb0[3] = {(float)di, (float)dj, (float)dk};
-        const float nb1[3] = {(float)di + 1.f, (float)dj + 1.f, (float)dk + 1.f};
-        if (box_gap2(cb0, cb1, nb0, nb1) <= cut) {
-          int32_t ln[3];
-          const int32_t key = key_at(c.nx, pl.cs, di, dj, dk, ln);
-          qa = cell_start[key];
-          qb = cell_start[key + 1];
-        }
-      }
-      uint64_t rest = __ballot(l < 27 && qa < qb);  // cells with records
-      // chunks of 64 records of the cells in turn; the next chunk is requested
-      // before the current one is used
-      int q = rest ? __ffsll((unsigned long long)rest) - 1 : 27;
-      int32_t s = q < 27 ? __builtin_amdgcn_readlane(qa, q) : 0;
-      float4 rn = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (q < 27 && s + l < __builtin_amdgcn_readlane(qb, q)) rn = rec[s + l];
-      while (q < 27) {
-        const int32_t b = __builtin_amdgcn_readlane(qb, q);
-        const int32_t m = min(64, b - s);
-        const float4 r = rn;
-        const int qc = q;
-        int32_t sn = s + 64;
-        if (sn >= b) {
-          rest &= ~(1ull << q);
-          q = rest ? __ffsll((unsigned long long)rest) - 1 : 27;
-          sn = q < 27 ? __builtin_amdgcn_readlane(qa, q) : 0;
-        }
-        if (q < 27 && sn + l < __builtin_amdgcn_readlane(qb, q)) rn = rec[sn + l];
-        s = sn;
-        int ei, ej, ek;
-        walk_off(qc, 1, &ei, &ej, &ek);
-        const int di = ci + ei, dj = cj + ej, dk = ck + ek;
-        // block-local coordinates of the walked cell (8 divides nx)
-        const int32_t lx = (pl.cs[0] + di) & 7, ly = (pl.cs[1] + dj) & 7,
-                      lz = (pl.cs[2] + dk) & 7;
-        float vx = 0.f, vy = 0.f, vz = 0.f;
-        if (l < m) {
-          vx = rel1(r.x, lx, di, 0.0f);
-          vy = rel1(r.y, ly, dj, 0.0f);
-          vz = rel1(r.z, lz, dk, 0.0f);
-        }
-        if (m >= 16) {
-          float nb0[3], nb1[3];
-          nb0[0] = wave_minf(l < m ? vx : INFINITY);
-          nb0[1] = wave_minf(l < m ? vy : INFINITY);
-          nb0[2] = wave_minf(l < m ? vz : INFINITY);
-          nb1[0] = wave_maxf(l < m ? vx : -INFINITY);
-          nb1[1] = wave_maxf(l < m ? vy : -INFINITY);
-          nb1[2] = wave_maxf(l < m ? vz : -INFINITY);
-          if (box_gap2(cb0, cb1, nb0, nb1) > cut) continue;
-        }
-        uint32_t part = 0;  // <= 64 terms of <= 2^24
-        // the chunk goes through this wave's LDS slot and is read back with
-        // wave-uniform addresses (broadcast reads, no lane moves)
-        nbuf[w][l] = make_float4(vx, vy, vz, 0.0f);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // m is uniform: a scalar trip count, and a select instead of a branch
-        const int32_t mu = __builtin_amdgcn_readfirstlane(m);
-#pragma unroll 4
-        for (int32_t u = 0; u < mu; u++) {
-          const float4 qv = nbuf[w][u];
-          const float dx = qv.x - px, dy = qv.y - py, dz = qv.z - pz;
-          const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-          const uint32_t term = sterm_fix(d2, c.eps2f);
-          part += d2 <= c.rs2f ? term : 0u;
-        }
-        acc += part;
-        npair += (uint64_t)m;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      }
-      uint64_t key = have ? (((uint64_t)(uint32_t)tk.y << 32) | (uint32_t)pidx)
-                          : ~0ull;
-      if (!have) acc = 0;
+      uint64_t acc = pilot_walk(c, rec, pl, P, nbuf[w], l, cut, rest, q, s,
+                                INT32_MAX, npair);
+      uint64_t key = P.have
+                         ? (((uint64_t)(uint32_t)tk.y << 32) | (uint32_t)P.pidx)
+                         : ~0ull;
+      if (!P.have) acc = 0;
       wave_argmax(acc, key);
       if (l == 0) {
         taskres[2 * (int64_t)t] = acc;
@@ -1074,6 +1138,89 @@ k_centre_pilot(SodCfg c, const int32_t *__restrict__ cell_start,
   if (l == 0 && st_done) {
     atomicAdd(&cstat[0], st_done);
     atomicAdd(&cstat[2], st_pairs);
+  }
+}
+
+// Pieces of the split pilot tasks, one wave per (task, piece): piece p of a
+// task's C chunks in reach (walk_off order of the cells) is chunks
+// [p * per, (p + 1) * per), per = ceil(C / PILOT_PIECES); its sums are
+// added to the task's per-candidate totals.
+__global__ void __launch_bounds__(256)
+k_centre_pilot_big(SodCfg c, const int32_t *__restrict__ cell_start,
+                   const float4 *__restrict__ rec,
+                   const HaloPlan *__restrict__ plan,
+                   const int32_t *__restrict__ large,
+                   const int4 *__restrict__ tasks,
+                   const int32_t *__restrict__ big,
+                   const int32_t *__restrict__ nbig,
+                   unsigned long long *__restrict__ pacc,
+                   unsigned long long *__restrict__ cstat,
+                   int32_t *__restrict__ wq) {
+  __shared__ float4 nbuf[4][64];  // one neighbour chunk per wave
+  const int l = threadIdx.x & 63,
+            w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int32_t nu = min(*nbig, PILOT_BIGCAP) * PILOT_PIECES;
+  const float cut = c.rs2f * 1.0001f + 1e-6f;  // conservative chunk culling
+  unsigned long long st_pairs = 0;
+  for (;;) {
+    const int32_t u = wave_next(wq);
+    if (u >= nu) break;
+    const int32_t b = u / PILOT_PIECES, p = u % PILOT_PIECES;
+    const int32_t t = big[b];
+    const int4 tk = tasks[t];
+    const HaloPlan pl = plan[large[tk.x]];
+    PilotSetup P;
+    pilot_setup(c, cell_start, rec, pl, tk, l, cut, P);
+    // chunks of each cell and their prefix over the cells
+    const int32_t nch = l < 27 ? (P.qb - P.qa + 63) >> 6 : 0;
+    int32_t C;
+    const int32_t ex = wave_excl(nch, &C);
+    const int32_t per = (C + PILOT_PIECES - 1) / PILOT_PIECES;
+    const int32_t k0 = p * per;
+    if (k0 >= C) continue;
+    // the cell holding chunk k0, and that chunk's first record
+    const uint64_t at = __ballot(nch > 0 && ex <= k0 && k0 < ex + nch);
+    const int q = __ffsll((unsigned long long)at) - 1;
+    const int32_t s = __builtin_amdgcn_readlane(P.qa, q) +
+                      (k0 - __builtin_amdgcn_readlane(ex, q)) * 64;
+    const uint64_t rest =
+        __ballot(l < 27 && P.qa < P.qb) & ~((1ull << q) - 1ull);
+    uint64_t npair = 0;
+    const uint64_t acc = pilot_walk(c, rec, pl, P, nbuf[w], l, cut, rest, q,
+                                    s, min(per, C - k0), npair);
+    if (P.have) atomicAdd(&pacc[(int64_t)b * 64 + l], (unsigned long long)acc);
+    if (l == 0) st_pairs += npair * (uint64_t)tk.w;
+  }
+  if (l == 0 && st_pairs) atomicAdd(&cstat[2], st_pairs);
+}
+
+// Results of the split pilot tasks, one wave per task: the best (S, key)
+// over its candidates' totals, folded into hbest.
+__global__ void __launch_bounds__(256)
+k_centre_pilot_fin(const float4 *__restrict__ rec,
+                   const int4 *__restrict__ tasks,
+                   const int32_t *__restrict__ big,
+                   const int32_t *__restrict__ nbig,
+                   const unsigned long long *__restrict__ pacc,
+                   uint64_t *__restrict__ taskres,
+                   unsigned long long *__restrict__ hbest,
+                   unsigned long long *__restrict__ cstat) {
+  const int l = threadIdx.x & 63;
+  const int32_t b = (int32_t)(blockIdx.x * 4 + (threadIdx.x >> 6));
+  if (b >= min(*nbig, PILOT_BIGCAP)) return;
+  const int32_t t = big[b];
+  const int4 tk = tasks[t];
+  const bool have = l < tk.w;
+  uint64_t acc = have ? pacc[(int64_t)b * 64 + l] : 0;
+  uint64_t key = have ? (((uint64_t)(uint32_t)tk.y << 32) |
+                         (uint32_t)__float_as_int(rec[tk.z + l].w))
+                      : ~0ull;
+  wave_argmax(acc, key);
+  if (l == 0) {
+    taskres[2 * (int64_t)t] = acc;
+    taskres[2 * (int64_t)t + 1] = key;
+    atomicMax(&hbest[tk.x], (unsigned long long)acc);
+    atomicAdd(&cstat[0], 1ull);
   }
 }
 
@@ -3598,6 +3745,17 @@ __global__ void k_copy_i32(const int32_t *__restrict__ a,
   if (blockIdx.x == 0 && threadIdx.x == 0) *b = *a;
 }
 
+// records in reach above which a pilot task is split; PSOD_PILOT_BIG
+// overrides the default for tests (at least 64)
+static int32_t pilot_big() {
+  int32_t v = 1 << 15;
+  if (const char *e = getenv("PSOD_PILOT_BIG")) {
+    const long long x = atoll(e);
+    if (x >= 64 && x <= INT32_MAX) v = (int32_t)x;
+  }
+  return v;
+}
+
 // Work counts of a pass.  Eager calls read them back on the host and size
 // every launch exactly.  A pass that psod_pass_graph captures cannot read
 // anything back: its launches are sized by bounds (the halo count, and a
@@ -3650,10 +3808,12 @@ static int centre_search(psod_ctx *ctx, const SodCfg &c,
   if (tcap == 0) return PSOD_OK;
   // tasks, their results, bounds, bounds per (cell, group) plus the
   // per-group totals (28 x 8 per task), the results of the groups (8 per
-  // task), the work list of groups and its length
+  // task), the work list of groups and its length, then the split pilot
+  // tasks and their per-candidate totals
   int4 *tasks = (int4 *)psod_scratch(
       ctx, S_TASKRES,
-      (size_t)tcap * (sizeof(int4) + 24 + 28 * 8 * 8 + 8 * 16 + 8 * 4) + 256);
+      (size_t)tcap * (sizeof(int4) + 24 + 28 * 8 * 8 + 8 * 16 + 8 * 4) + 256 +
+          (size_t)PILOT_BIGCAP * (4 + 64 * 8));
   if (!tasks) return PSOD_E_NOMEM;
   uint64_t *tres = (uint64_t *)(tasks + tcap);
   unsigned long long *ubt = (unsigned long long *)(tres + 2 * (int64_t)tcap);
@@ -3661,7 +3821,13 @@ static int centre_search(psod_ctx *ctx, const SodCfg &c,
   uint64_t *ires = (uint64_t *)(cub + (int64_t)tcap * 28 * 8);
   int32_t *items = (int32_t *)(ires + 16 * (int64_t)tcap);
   int32_t *nitems = items + 8 * (int64_t)tcap;
-  int32_t *wq = nitems + 4;  // work counters of the bound, pilot, group
+  // work counters of the bound, pilot, group and split-pilot launches, and
+  // the count of split pilot tasks
+  int32_t *wq = nitems + 4;
+  int32_t *nbig = wq + 4;
+  // split pilot tasks and their per-candidate totals
+  int32_t *big = nitems + 64;
+  unsigned long long *pacc = (unsigned long long *)(big + PILOT_BIGCAP);
   k_task_emit<<<grid_for(nlarge, 256), 256, 0, st>>>(
       c, cat->cell_start, plan, large, nlarge, dn, ntask, task0,
       dn ? overflow : nullptr, tasks);
@@ -3676,7 +3842,7 @@ static int centre_search(psod_ctx *ctx, const SodCfg &c,
   unsigned long long *cstat = hbest + nlarge;
   unsigned long long *hpil = cstat + 4;
   PSOD_TRY(psod_fill_u64(ctx, (uint64_t *)hbest, 2 * (int64_t)nlarge + 4, 0));
-  PSOD_TRY(psod_fill_i32(ctx, nitems, 8, 0));
+  PSOD_TRY(psod_fill_i32(ctx, nitems, 12, 0));
   if (!ctx->capturing) PSOD_HIP(hipEventRecord(ctx->ev[0], st));
   // as many workgroups as fit at once; their waves take work from counters
   auto resident = [&](const void *k) {
@@ -3694,7 +3860,15 @@ static int centre_search(psod_ctx *ctx, const SodCfg &c,
   PSOD_LAUNCH_CHECK();
   k_centre_pilot<<<resident((const void *)k_centre_pilot), 256, 0, st>>>(
       c, cat->cell_start, (const float4 *)cat->rec, plan, large, tasks, ntot,
-      tres, hbest, hpil, ubt, cstat, wq + 1);
+      tres, hbest, hpil, ubt, cstat, pilot_big(), big, nbig, pacc, wq + 1);
+  PSOD_LAUNCH_CHECK();
+  k_centre_pilot_big<<<resident((const void *)k_centre_pilot_big), 256, 0,
+                       st>>>(c, cat->cell_start, (const float4 *)cat->rec,
+                             plan, large, tasks, big, nbig, pacc, cstat,
+                             wq + 3);
+  PSOD_LAUNCH_CHECK();
+  k_centre_pilot_fin<<<PILOT_BIGCAP / 4, 256, 0, st>>>(
+      (const float4 *)cat->rec, tasks, big, nbig, pacc, tres, hbest, cstat);
   PSOD_LAUNCH_CHECK();
   k_task_items<<<grid_for(tcap, 256), 256, 0, st>>>(
       tasks, ntot, tcap, ubt, hpil, cub, hbest, items, nitems, cstat);

@@ -257,6 +257,76 @@ def test_sod_retry_with_larger_slices(ctx, c64, monkeypatch):
         np.testing.assert_array_equal(a[k], b[k], err_msg=k)
 
 
+def test_staged_giants_retry_with_larger_slices(ctx, c64, monkeypatch):
+    """Giant halos grown by the staged launches whose bands overflow their
+    collection slice go to the re-run launch with larger slices; the
+    results are those of one workgroup per halo with the default slice."""
+    import psod_amd as P
+    nx, N = c64["nx"], c64["N"]
+    cat = ctx.cell_sort(c64["pos"], nx, labels=c64["lab"])
+    grid = ctx.cic_deposit(cat, nx)
+    seeds = ctx.seeds(cat, grid, c64["nh"])
+    nmem = ctx.label_counts(c64["lab"], c64["nh"])
+    p = P.sod_params(nx, N, deltas=(200.0, 500.0), delta2=800.0,
+                     velocity=True)
+
+    def run():
+        o = ctx.sod_fof(p, cat, c64["pos"], seeds, nmem, vel=c64["vel"],
+                        ids=c64["ids"])
+        return {k: host(v) for k, v in o.items()}, ctx.sod_stats()
+
+    a, sa = run()
+    try:
+        ctx.set_tunables(giant_min=1)
+        monkeypatch.setenv("PSOD_SLICE_KEYS", "16")
+        b, sb = run()
+    finally:
+        monkeypatch.delenv("PSOD_SLICE_KEYS", raising=False)
+        ctx.set_tunables()
+    print("stats", sa, sb, flush=True)
+    assert sa["giants"] == 0 and sb["giants"] > 0
+    assert sa["retried"] == 0 and sb["retried"] > 0
+    for k in a:
+        if k == "com_vel":
+            # the velocity sums follow the order of a path's walk
+            np.testing.assert_allclose(a[k], b[k], rtol=1e-12, atol=1e-12)
+        elif k in ("rint", "mint", "overdens", "overdens2"):
+            np.testing.assert_allclose(a[k], b[k], rtol=1e-13, err_msg=k)
+        else:
+            np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+
+
+def test_split_pilot_tasks_give_the_same_centres(ctx, c64, monkeypatch):
+    """Pilot tasks split into pieces summed by separate waves (here every
+    task with more than 64 records in reach, up to the split list's
+    capacity) give the same integer sums, hence the same centres and
+    results."""
+    import psod_amd as P
+    nx, N = c64["nx"], c64["N"]
+    cat = ctx.cell_sort(c64["pos"], nx, labels=c64["lab"])
+    grid = ctx.cic_deposit(cat, nx)
+    seeds = ctx.seeds(cat, grid, c64["nh"])
+    nmem = ctx.label_counts(c64["lab"], c64["nh"])
+    p = P.sod_params(nx, N, deltas=(200.0, 500.0), delta2=800.0,
+                     velocity=True)
+
+    def run():
+        o = ctx.sod_fof(p, cat, c64["pos"], seeds, nmem, vel=c64["vel"],
+                        ids=c64["ids"])
+        return {k: host(v) for k, v in o.items()}, ctx.sod_stats()
+
+    a, sa = run()
+    try:
+        monkeypatch.setenv("PSOD_PILOT_BIG", "64")
+        b, sb = run()
+    finally:
+        monkeypatch.delenv("PSOD_PILOT_BIG", raising=False)
+    print("stats", sa, sb, flush=True)
+    assert sa["centre_tasks_all"] == sb["centre_tasks_all"] > 0
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+
+
 def test_whole_pass_matches_the_chain(ctx, c64):
     """psod_pass gives exactly the outputs of the step-by-step chain."""
     import psod_amd as P
