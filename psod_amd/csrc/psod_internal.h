@@ -72,6 +72,7 @@ enum ScratchSlot {
   S_GIANT_H,   // SOD: their histogram slices
   S_GIANT_P,   // SOD: their collection slices
   S_GIANT_SEG, // SOD: per-segment counts of their cubes, prefix sums
+  S_GIANT_V,   // SOD: values of the device-wide sorts of their bands
   S_NSLOT
 };
 

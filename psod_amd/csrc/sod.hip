@@ -326,9 +326,9 @@ k_sod_small(SodCfg c, const int32_t *__restrict__ cell_start,
       int32_t lc[3];
       key_at(c.nx, pl.cs, di, dj, dk, lc);
       float4 v;
-      v.x = rel1(r.x, lc[0], di, 0.0f);
-      v.y = rel1(r.y, lc[1], dj, 0.0f);
-      v.z = rel1(r.z, lc[2], dk, 0.0f);
+      v.x = cell_rel(r.x, lc[0], di);
+      v.y = cell_rel(r.y, lc[1], dj);
+      v.z = cell_rel(r.z, lc[2], dk);
       v.w = r.w;
       S.a.nb[j] = v;
       const bool inner = di >= -1 && di <= 1 && dj >= -1 && dj <= 1 &&
@@ -647,20 +647,31 @@ k_task_emit(SodCfg c, const int32_t *__restrict__ cell_start,
 }
 
 // Minimum / maximum over the 64 lanes of a wave, returned wave-uniform.  All
-// lanes must be active.  Row shifts by 1, 2, 4, 8 leave each 16-lane row's
-// result in its lane 15; broadcasts of lanes 15 and 31 fold the rows into
-// lane 63 (register moves between lanes, no LDS round trips).
+// lanes must be active.  The floats are mapped to integers of the same
+// order (negative values: magnitude bits flipped), so each step is one
+// integer min/max with a DPP operand (the float forms also quiet NaNs,
+// an extra instruction per step).  Row shifts by 1, 2, 4, 8 leave each
+// 16-lane row's result in its lane 15; broadcasts of lanes 15 and 31 fold
+// the rows into lane 63 (register moves between lanes, no LDS round trips).
+__device__ inline int32_t f2ord(float f) {
+  const int32_t i = __float_as_int(f);
+  return i ^ ((i >> 31) & 0x7fffffff);
+}
+__device__ inline float ord2f(int32_t i) {
+  return __int_as_float(i ^ ((i >> 31) & 0x7fffffff));
+}
 template <bool MAX>
-__device__ inline float wave_extf(float v) {
-  const int id = __float_as_int(MAX ? -INFINITY : INFINITY);
-  auto op = [](float a, float b) { return MAX ? fmaxf(a, b) : fminf(a, b); };
-  v = op(v, __int_as_float(__builtin_amdgcn_update_dpp(id, __float_as_int(v), 0x111, 0xf, 0xf, false)));
-  v = op(v, __int_as_float(__builtin_amdgcn_update_dpp(id, __float_as_int(v), 0x112, 0xf, 0xf, false)));
-  v = op(v, __int_as_float(__builtin_amdgcn_update_dpp(id, __float_as_int(v), 0x114, 0xf, 0xf, false)));
-  v = op(v, __int_as_float(__builtin_amdgcn_update_dpp(id, __float_as_int(v), 0x118, 0xf, 0xf, false)));
-  v = op(v, __int_as_float(__builtin_amdgcn_update_dpp(id, __float_as_int(v), 0x142, 0xa, 0xf, false)));
-  v = op(v, __int_as_float(__builtin_amdgcn_update_dpp(id, __float_as_int(v), 0x143, 0xc, 0xf, false)));
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+__device__ inline float wave_extf(float x) {
+  const int id = MAX ? INT32_MIN : INT32_MAX;
+  auto op = [](int32_t a, int32_t b) { return MAX ? max(a, b) : min(a, b); };
+  int32_t v = f2ord(x);
+  v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x111, 0xf, 0xf, false));
+  v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x112, 0xf, 0xf, false));
+  v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x114, 0xf, 0xf, false));
+  v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x118, 0xf, 0xf, false));
+  v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x142, 0xa, 0xf, false));
+  v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x143, 0xc, 0xf, false));
+  return ord2f(__builtin_amdgcn_readlane(v, 63));
 }
 __device__ inline float wave_minf(float v) { return wave_extf<false>(v); }
 __device__ inline float wave_maxf(float v) { return wave_extf<true>(v); }
@@ -777,9 +788,9 @@ k_centre_bound(SodCfg c, const int32_t *__restrict__ cell_start,
       const bool have = l < tk.w;
       if (have) {
         const float4 r = rec[tk.z + l];
-        px = rel1(r.x, lc[0], ci, 0.0f);
-        py = rel1(r.y, lc[1], cj, 0.0f);
-        pz = rel1(r.z, lc[2], ck, 0.0f);
+        px = cell_rel(r.x, lc[0], ci);
+        py = cell_rel(r.y, lc[1], cj);
+        pz = cell_rel(r.z, lc[2], ck);
       }
       float cb0[3], cb1[3];
       cb0[0] = wave_minf(have ? px : INFINITY);
@@ -804,6 +815,8 @@ k_centre_bound(SodCfg c, const int32_t *__restrict__ cell_start,
       }
       // minima in scalar registers, maxima in vector registers: the clamp
       // (v_med3) takes one scalar operand, so neither needs a move per term
+      const float hb[3] = {__shfl(cb1[0], 0, 64), __shfl(cb1[1], 0, 64),
+                           __shfl(cb1[2], 0, 64)};
       float B[4][6];
 #pragma unroll
       for (int g = 0; g < 4; g++) {
@@ -871,14 +884,15 @@ k_centre_bound(SodCfg c, const int32_t *__restrict__ cell_start,
         float vx = 0.f, vy = 0.f, vz = 0.f;
         bool near = false;
         if (valid) {
-          vx = rel1(r.x, lx, di, 0.0f);
-          vy = rel1(r.y, ly, dj, 0.0f);
-          vz = rel1(r.z, lz, dk, 0.0f);
-          // within reach of the task's box (its groups lie inside it)
-          const float ex = fmaxf(0.0f, fmaxf(cb0[0] - vx, vx - cb1[0]));
-          const float ey = fmaxf(0.0f, fmaxf(cb0[1] - vy, vy - cb1[1]));
-          const float ez = fmaxf(0.0f, fmaxf(cb0[2] - vz, vz - cb1[2]));
-          near = ex * ex + ey * ey + ez * ez <= cut;
+          vx = cell_rel(r.x, lx, di);
+          vy = cell_rel(r.y, ly, dj);
+          vz = cell_rel(r.z, lz, dk);
+          // within reach of the task's box (its groups lie inside it): the
+          // coordinate minus its clamp to the box
+          const float ex = vx - __builtin_amdgcn_fmed3f(vx, cb0[0], hb[0]);
+          const float ey = vy - __builtin_amdgcn_fmed3f(vy, cb0[1], hb[1]);
+          const float ez = vz - __builtin_amdgcn_fmed3f(vz, cb0[2], hb[2]);
+          near = fmaf(ex, ex, fmaf(ey, ey, ez * ez)) <= cut;
         }
         if (near) {
 #pragma unroll
@@ -951,9 +965,9 @@ __device__ __forceinline__ void pilot_setup(const SodCfg &c,
   P.have = l < tk.w;
   if (P.have) {
     const float4 r = rec[tk.z + l];
-    P.px = rel1(r.x, lc[0], P.ci, 0.0f);
-    P.py = rel1(r.y, lc[1], P.cj, 0.0f);
-    P.pz = rel1(r.z, lc[2], P.ck, 0.0f);
+    P.px = cell_rel(r.x, lc[0], P.ci);
+    P.py = cell_rel(r.y, lc[1], P.cj);
+    P.pz = cell_rel(r.z, lc[2], P.ck);
     P.pidx = __float_as_int(r.w);
   }
   P.cb0[0] = wave_minf(P.have ? P.px : INFINITY);
@@ -1018,9 +1032,9 @@ __device__ __forceinline__ uint64_t pilot_walk(const SodCfg &c,
                   lz = (pl.cs[2] + dk) & 7;
     float vx = 0.f, vy = 0.f, vz = 0.f;
     if (l < m) {
-      vx = rel1(r.x, lx, di, 0.0f);
-      vy = rel1(r.y, ly, dj, 0.0f);
-      vz = rel1(r.z, lz, dk, 0.0f);
+      vx = cell_rel(r.x, lx, di);
+      vy = cell_rel(r.y, ly, dj);
+      vz = cell_rel(r.z, lz, dk);
     }
     if (m >= 16) {
       float nb0[3], nb1[3];
@@ -1362,9 +1376,9 @@ k_centre_group(SodCfg c, const int32_t *__restrict__ cell_start,
       int32_t pidx = 0;
       if (kc < nc) {
         const float4 r = rec[tk.z + 8 * g + kc];
-        px = rel1(r.x, lc[0], ci, 0.0f);
-        py = rel1(r.y, lc[1], cj, 0.0f);
-        pz = rel1(r.z, lc[2], ck, 0.0f);
+        px = cell_rel(r.x, lc[0], ci);
+        py = cell_rel(r.y, lc[1], cj);
+        pz = cell_rel(r.z, lc[2], ck);
         pidx = __float_as_int(r.w);
       }
       float gb0[3], gb1[3];  // the item's box
@@ -1374,6 +1388,9 @@ k_centre_group(SodCfg c, const int32_t *__restrict__ cell_start,
       gb1[0] = wave_maxf(kc < nc ? px : -INFINITY);
       gb1[1] = wave_maxf(kc < nc ? py : -INFINITY);
       gb1[2] = wave_maxf(kc < nc ? pz : -INFINITY);
+      // the maxima in vector registers: v_med3 takes one scalar operand
+      const float hb[3] = {__shfl(gb1[0], 0, 64), __shfl(gb1[1], 0, 64),
+                           __shfl(gb1[2], 0, 64)};
       // record ranges of the 27 cells nearest first and the item's bound in
       // each, one per lane; a cell beyond reach of the item's box is empty
       // (its bound is 0: none of its records is within reach of the group)
@@ -1452,13 +1469,14 @@ k_centre_group(SodCfg c, const int32_t *__restrict__ cell_start,
         bool near = false;
         float vx = 0.f, vy = 0.f, vz = 0.f;
         if (l < m) {
-          vx = rel1(r.x, lx, di, 0.0f);
-          vy = rel1(r.y, ly, dj, 0.0f);
-          vz = rel1(r.z, lz, dk, 0.0f);
-          const float ex = fmaxf(0.0f, fmaxf(gb0[0] - vx, vx - gb1[0]));
-          const float ey = fmaxf(0.0f, fmaxf(gb0[1] - vy, vy - gb1[1]));
-          const float ez = fmaxf(0.0f, fmaxf(gb0[2] - vz, vz - gb1[2]));
-          near = ex * ex + ey * ey + ez * ez <= cut;
+          vx = cell_rel(r.x, lx, di);
+          vy = cell_rel(r.y, ly, dj);
+          vz = cell_rel(r.z, lz, dk);
+          // the coordinate minus its clamp to the item's box
+          const float ex = vx - __builtin_amdgcn_fmed3f(vx, gb0[0], hb[0]);
+          const float ey = vy - __builtin_amdgcn_fmed3f(vy, gb0[1], hb[1]);
+          const float ez = vz - __builtin_amdgcn_fmed3f(vz, gb0[2], hb[2]);
+          near = fmaf(ex, ex, fmaf(ey, ey, ez * ez)) <= cut;
         }
         const uint64_t nm = __ballot(near);
         const int nn = __popcll(nm);
@@ -2398,11 +2416,12 @@ __device__ inline int32_t grow_bins(const SodCfg &c, GrowShared &G,
 
 // cross, on the keys of walk 2 in pool: GS_PROFILE, GS_BEGIN (larger
 // reach) or GS_FAILED (G.need slots beyond pool_cap)
+// Ranges r with bit r of `presorted` set were sorted before the launch.
 __device__ inline int32_t grow_cross(const SodCfg &c, GrowShared &G,
                                      const int32_t *hist, uint64_t *pool,
                                      const float *pos, GrowVars &V,
                                      HaloResult &res, const double *xcs,
-                                     int64_t pool_cap) {
+                                     int64_t pool_cap, uint32_t presorted) {
   const int32_t *C = hist;
   const int32_t nb = V.nb;
   const float scale = V.scale;
@@ -2410,7 +2429,8 @@ __device__ inline int32_t grow_cross(const SodCfg &c, GrowShared &G,
   const double Wd = (double)V.W, W3 = Wd * Wd * Wd;
   const int nr = G.nr;
   for (int r = 0; r < nr; r++)
-    block_sort_pool(pool + G.off[r], (int32_t)(G.k1[r] - G.k0[r]), G);
+    if (!((presorted >> r) & 1))
+      block_sort_pool(pool + G.off[r], (int32_t)(G.k1[r] - G.k0[r]), G);
   // key of rank k (1-based) among the collected ranges
   auto key_rank = [&](int64_t k) {
     int r = 0;
@@ -3194,6 +3214,7 @@ struct GiantState {
   unsigned long long ncube;  // particles of the cube (the count walk)
   int64_t need, bump;
   int32_t nr;
+  uint32_t presorted;  // ranges sorted by the device-wide sort (bit r)
   int32_t fill[MAXRANGE], rs[MAXRANGE], re[MAXRANGE];
   int64_t off[MAXRANGE], k0[MAXRANGE], k1[MAXRANGE];
   int32_t sg_s[PSOD_MAXDELTA][SEGCAP], sg_e[PSOD_MAXDELTA][SEGCAP],
@@ -3590,6 +3611,39 @@ k_giant_bins(SodCfg c, GiantState *__restrict__ S,
   giant_save(G, T, V, step, nch);
 }
 
+// Band ranges of the staged giants with at least sort_min keys (2^16
+// unless PSOD_GIANT_SORT_MIN says otherwise) are sorted by the device-wide
+// radix sort before k_giant_cross, at most GIANT_SORT_LIST per cycle: one
+// workgroup's bitonic network over such a range (2^20 keys, 36 passes
+// through global memory) took most of a 24 ms cross.
+constexpr int GIANT_SORT_LIST = 64;
+static_assert(MAXRANGE <= 32, "presorted ranges are a 32-bit mask");
+
+// The ranges to sort: list[0] = their count, then (first pool slot, keys)
+// per range; marks them in each giant's presorted mask (one thread).
+__global__ void k_giant_sortlist(GiantState *__restrict__ S, int32_t ngiant,
+                                 int64_t pool_cap, int64_t sort_min,
+                                 int64_t *__restrict__ list) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  int64_t k = 0;
+  for (int32_t g = 0; g < ngiant; g++) {
+    GiantState &T = S[g];
+    uint32_t mask = 0;
+    if (T.step == GS_COLLECT)
+      for (int r = 0; r < T.nr; r++) {
+        const int64_t n = T.k1[r] - T.k0[r];
+        if (n >= sort_min && k < GIANT_SORT_LIST) {
+          list[1 + 2 * k] = (int64_t)g * pool_cap + T.off[r];
+          list[2 + 2 * k] = n;
+          mask |= 1u << r;
+          k++;
+        }
+      }
+    T.presorted = mask;
+  }
+  list[0] = k;
+}
+
 // cross for the giants in GS_COLLECT; clears the sums of walk 3 and the
 // counter of its launch
 __global__ void __launch_bounds__(GL_T)
@@ -3613,7 +3667,7 @@ k_giant_cross(SodCfg c, GiantState *__restrict__ S,
   HaloResult res = T.res;
   double xcs[3] = {T.xcs[0], T.xcs[1], T.xcs[2]};
   const int32_t step =
-      grow_cross(c, G, hist, pool, pos, V, res, xcs, pool_cap);
+      grow_cross(c, G, hist, pool, pos, V, res, xcs, pool_cap, T.presorted);
   __syncthreads();
   if (step == GS_FAILED)
     grow_close(c, counters, retry, needmax, G.need, V, res, true, out);
@@ -3920,6 +3974,7 @@ struct SodScratch {
   int32_t *giants;     // (e, W) of the giant halos left to the staged growth
   int64_t *glay;       // their per-segment count layout (k_giant_layout)
   int32_t *gchunk;     // prefix of their chunks in a walk (k_giant_chunks)
+  int64_t *gsort;      // their band ranges for the device-wide sort
   GiantState *gstate;  // their states, GIANT_CAP entries
 };
 
@@ -3936,13 +3991,16 @@ static int sod_scratch(psod_ctx *ctx, int64_t nhalo, SodScratch *S) {
   // (GIANT_CAP + 1), states (GIANT_CAP), each 256-byte aligned
   constexpr size_t G_LIST = 0, G_LAY = 2 * GIANT_CAP * 4,
                    G_CH = G_LAY + ((GIANT_CAP + 1) * 8 + 255) / 256 * 256,
-                   G_ST = G_CH + ((GIANT_CAP + 1) * 4 + 255) / 256 * 256;
+                   G_SORT = G_CH + ((GIANT_CAP + 1) * 4 + 255) / 256 * 256,
+                   G_ST = G_SORT +
+                          ((2 * GIANT_SORT_LIST + 1) * 8 + 255) / 256 * 256;
   char *gb = (char *)psod_scratch(
       ctx, S_GIANT, G_ST + (size_t)GIANT_CAP * sizeof(GiantState));
   if (!S->plan || !lists || !ntask || !gb) return PSOD_E_NOMEM;
   S->giants = (int32_t *)(gb + G_LIST);
   S->glay = (int64_t *)(gb + G_LAY);
   S->gchunk = (int32_t *)(gb + G_CH);
+  S->gsort = (int64_t *)(gb + G_SORT);
   S->gstate = (GiantState *)(gb + G_ST);
   S->small = lists;
   S->large = lists + nhalo;
@@ -4184,6 +4242,11 @@ static int grow_giants(psod_ctx *ctx, const SodCfg &c,
     PSOD_LAUNCH_CHECK();
     return PSOD_OK;
   };
+  int64_t sort_min = 1 << 16;
+  if (const char *e = getenv("PSOD_GIANT_SORT_MIN")) {
+    const long long v = atoll(e);
+    if (v >= 2) sort_min = v;
+  }
   constexpr int MAX_CYCLES = 16;  // the reach at least doubles per cycle
   int cyc = 0;
   for (;; cyc++) {
@@ -4215,6 +4278,21 @@ static int grow_giants(psod_ctx *ctx, const SodCfg &c,
                                           S.retry, needmax, *out, wq);
     PSOD_LAUNCH_CHECK();
     PSOD_TRY(walk(GS_COLLECT, wq + 2));
+    // the large band ranges sorted across the GPU
+    k_giant_sortlist<<<1, 64, 0, st>>>(gs, ngiant, pool_cap, sort_min,
+                                       S.gsort);
+    PSOD_LAUNCH_CHECK();
+    int64_t sl[2 * GIANT_SORT_LIST + 1];
+    PSOD_HIP(hipMemcpyAsync(sl, S.gsort, sizeof(sl), hipMemcpyDeviceToHost,
+                            st));
+    PSOD_HIP(hipStreamSynchronize(st));
+    for (int64_t k = 0; k < sl[0]; k++) {
+      const int64_t n = sl[2 + 2 * k];
+      uint32_t *vals =
+          (uint32_t *)psod_scratch(ctx, S_GIANT_V, (size_t)n * 4);
+      if (!vals) return PSOD_E_NOMEM;
+      PSOD_TRY(psod_radix_sort_u64(ctx, n, pool + sl[1 + 2 * k], vals, 64));
+    }
     k_giant_cross<<<ngiant, GL_T, 0, st>>>(c, gs, hist, pool, pool_cap,
                                            cat->pos, S.counters, S.retry,
                                            needmax, *out, wq);

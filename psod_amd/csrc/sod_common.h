@@ -100,6 +100,14 @@ __device__ inline float rel1(float u, int32_t loc, int dcell, float fref) {
   return (float)dcell + ((u - (float)loc) - fref);
 }
 
+// The same with fref = 0, written u + (dcell - loc): the record lies in its
+// cell loc of the block (u in [loc, loc + 1), or loc = 0), so u - loc is
+// exact and both forms round the same real number once.  This one costs a
+// conversion of the wave-uniform offset and one add per record.
+__device__ inline float cell_rel(float u, int32_t loc, int dcell) {
+  return u + (float)(dcell - loc);
+}
+
 __device__ inline double rel1d(float u, int32_t loc, int dcell, double fref) {
   return (double)dcell + (((double)u - (double)loc) - fref);
 }

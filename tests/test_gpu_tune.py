@@ -101,12 +101,13 @@ def test_autotune_file_round_trip(ctx, tmp_path):
         c2.close()
 
 
-def test_staged_growth_of_giant_halos(ctx):
+def test_staged_growth_of_giant_halos(ctx, monkeypatch):
     """A 128^3 catalogue with a 1.5e6-particle central clump: halos grown
     by the staged launches (giant_min at or below their cubes; at 4096 more
     halos qualify than the 256 staged per pass, so the rest stay on one
     workgroup each) give the results of one workgroup per halo (giant_min
-    above every cube)."""
+    above every cube), also with every band range of 2 or more keys sorted
+    by the device-wide sort instead of the giant's workgroup."""
     import psod_amd as P
     nx = 128
     N = nx ** 3
@@ -117,11 +118,19 @@ def test_staged_growth_of_giant_halos(ctx):
     p = P.sod_params(nx, N, deltas=(200.0, 500.0), delta2=800.0,
                      velocity=True)
     runs = []
-    for gm in (1 << 30, 1 << 20, 4096):
-        ctx.set_tunables(giant_min=gm)
-        o = ctx.run_pass(p, pos, lab, nh, vel=vel, ids=ids)
-        runs.append({k: v.cpu().numpy() for k, v in o.items()})
-    ctx.set_tunables()
+    try:
+        for gm, smin in ((1 << 30, None), (1 << 20, None), (4096, None),
+                         (4096, "2")):
+            if smin:
+                monkeypatch.setenv("PSOD_GIANT_SORT_MIN", smin)
+            ctx.set_tunables(giant_min=gm)
+            o = ctx.run_pass(p, pos, lab, nh, vel=vel, ids=ids)
+            runs.append({k: v.cpu().numpy() for k, v in o.items()})
+            st = ctx.sod_stats()
+            assert (st["giants"] > 0) == (gm < 1 << 30), (gm, st)
+    finally:
+        monkeypatch.delenv("PSOD_GIANT_SORT_MIN", raising=False)
+        ctx.set_tunables()
     o0 = runs[0]
     assert np.all(o0["status"] & 1)
     assert o0["count"][:, 0].max() > 500_000
