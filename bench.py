@@ -55,6 +55,9 @@ def parse():
     ap.add_argument("--giant-min", type=int, default=0,
                     help="growth cube particles from which a halo is grown by "
                          "the staged launches (0: the context's tunable)")
+    ap.add_argument("--tunable", action="append", default=[],
+                    metavar="NAME=VALUE",
+                    help="set one of the context's tunables (repeatable)")
     ap.add_argument("--tune-file", default=None,
                     help="tuning file the autotuner appends to")
     ap.add_argument("--no-plain", action="store_true")
@@ -273,6 +276,12 @@ def main():
     tuned = None
     if args.giant_min > 0:
         ctx.set_tunables(giant_min=args.giant_min)
+    if args.tunable:
+        cur = ctx.tunables(nx)
+        for kv in args.tunable:
+            k, v = kv.split("=")
+            cur[k] = int(v)
+        ctx.set_tunables(**cur)
     if args.tune:
         t0 = time.perf_counter()
         chosen, dep_ms, sod_ms = ctx.autotune(p, pos, lab, nh, vel=vel,

@@ -73,7 +73,7 @@ int psod_sod_stats(psod_ctx *ctx, int64_t *out, int32_t n);
  *   small_max        particles in the 5^3 cells around the seed up to which a
  *                    halo is done by one wavefront [1024, at most 1024]
  *   bound_min        particles in the 5^3 cells from which the centre search
- *                    bounds its boxes of 16 candidates before summing [2048]
+ *                    bounds its boxes of 16 candidates before summing [1024]
  *   dense_tile       own particles from which a CIC tile is dispatched first
  *                    [16384]
  *   small_wg_per_cu  one-wavefront halo workgroups per CU [8, at most 32]

@@ -84,7 +84,7 @@ struct TuneEntry {
 };
 
 // built-in defaults of the tunables
-constexpr int32_t TUNE_SMALL_MAX = 1024, TUNE_BOUND_MIN = 2048,
+constexpr int32_t TUNE_SMALL_MAX = 1024, TUNE_BOUND_MIN = 1024,
                   TUNE_DENSE_TILE = 1 << 14, TUNE_SMALL_WG = 8,
                   TUNE_GIANT_MIN = 1 << 20;
 

@@ -694,8 +694,10 @@ __constant__ int8_t kCellOrder[27] = {13, 4,  10, 12, 14, 16, 22, 1,  3,
                                       25, 0,  2,  6,  8,  18, 20, 24, 26};
 
 // Halos with fewer than SodCfg::bound_min particles in the 5^3 cells around
-// the seed (a tunable, 2048 by default) skip the bound: bounding costs about
-// as much as their exact sums.
+// the seed (a tunable, 1024 by default) skip the bound: bounding costs about
+// as much as their exact sums.  With the items abandoned cell by cell,
+// 1024 gave 140.6 ms of SOD at c3 against 144.2 (2048), 141.6 (512) and
+// 149.8 ms (4096).
 
 // Upper bound of the term of any pair at squared distance >= g2 within the
 // cutoff.  The float d2 of a pair is at least g2 up to a few ulps, hence the
@@ -1306,23 +1308,20 @@ __global__ void k_item_keys(const int32_t *__restrict__ items,
   vals[i] = (uint32_t)it;
 }
 
-// Shell of the cell at position qi of the nearest-first order: the cell
-// itself, the 6 faces, the 12 edges, the 8 corners.
-__device__ inline int cell_shell(int qi) {
-  return qi == 0 ? 0 : qi < 7 ? 1 : qi < 19 ? 2 : 3;
-}
-
 // Exact phase, one wave per item (task t, candidates 8g..8g+15).  Lane l
 // takes candidate l & 15.  The neighbours within reach of the item's box are
 // queued in LDS across chunks and cells and taken 64 at a time, neighbour
 // u + (l >> 4) by lane l in steps of 4, so that every step evaluates 64
 // pairs.  Each pair's term comes from the same operands and operations as in
 // the pilots (neighbour minus candidate, the same fused squares), so S is
-// the same integer.  After each shell of cells (nearest first) the queue is
-// drained and the item is abandoned when no candidate's exact sum so far
-// plus the bound of the cells left reaches the halo's best.  Items of 8
-// candidates (the bound's boxes were of 8 then) took 236 ms of SOD at c3
-// against 193 ms with boxes and items of 16.
+// the same integer.  After each cell (nearest first: the cell itself, the 6
+// faces, the 12 edges, the 8 corners) the queue is drained and the item is
+// abandoned when no candidate's exact sum so far plus the bound of the
+// cells left reaches the halo's best.  A drain pads at most 3 entries, so
+// checking after every cell rather than every shell of cells costs little
+// and abandons earlier (44.4 -> 39.7 ms at 1024^3, 8% fewer pair tests).
+// Items of 8 candidates (the bound's boxes were of 8 then) took 236 ms of
+// SOD at c3 against 193 ms with boxes and items of 16.
 // 5 waves per SIMD (94 VGPRs, a few spilled): 48.2 -> 46.5 ms at 1024^3
 __global__ void __launch_bounds__(256, 5)
 k_centre_group(SodCfg c, const int32_t *__restrict__ cell_start,
@@ -1483,8 +1482,8 @@ k_centre_group(SodCfg c, const int32_t *__restrict__ cell_start,
         if (near) gq[w][qlen + __popcll(nm & below)] = make_float4(vx, vy, vz, 0.f);
         qlen += nn;
         npair += (uint64_t)nn;
-        const bool shell_end = qn == 27 || cell_shell(qn) != cell_shell(qi);
-        if (qlen >= 64 || (shell_end && qlen > 0)) {
+        const bool cell_end = qn != qi;
+        if (qlen >= 64 || (cell_end && qlen > 0)) {
           __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
           __builtin_amdgcn_wave_barrier();
           __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -1502,7 +1501,7 @@ k_centre_group(SodCfg c, const int32_t *__restrict__ cell_start,
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             if (l < qlen) gq[w][l] = mv;
           }
-          if (shell_end && qlen > 0) {
+          if (cell_end && qlen > 0) {
             // at most 63 left: pad to a multiple of 8 and take them too
             const int ns = (qlen + 3) >> 2;
             if (l >= qlen && l < 4 * ns)
@@ -1518,7 +1517,7 @@ k_centre_group(SodCfg c, const int32_t *__restrict__ cell_start,
           __builtin_amdgcn_wave_barrier();
           __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
-        if (shell_end && qn < 27) {
+        if (cell_end && qn < 27) {
           // every cell before qn is summed, the cells from qn on are bounded
           acc64 += part;
           part = 0;

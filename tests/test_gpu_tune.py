@@ -24,7 +24,7 @@ def test_tunables_do_not_change_results(ctx):
     P, p, pos, vel, lab, ids, nh = _pass(ctx)
     nx = p.nx
     base = ctx.tunables(nx)
-    assert base == dict(small_max=1024, bound_min=2048, dense_tile=16384,
+    assert base == dict(small_max=1024, bound_min=1024, dense_tile=16384,
                         small_wg_per_cu=8, giant_min=1 << 20)
     runs = []
     for t in (dict(), dict(small_max=256, bound_min=64, dense_tile=512,
