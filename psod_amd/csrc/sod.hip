@@ -859,27 +859,32 @@ k_centre_bound(SodCfg c, const int32_t *__restrict__ cell_start,
 #pragma unroll
       for (int g = 0; g < 4; g++) ubg[g] = 0;
       int nch = 0;  // chunks summed into ubg
-      // chunks of 64 records of the cells in turn; the next chunk is requested
-      // before the current one is used
-      int q = rest ? __ffsll((unsigned long long)rest) - 1 : 27;
-      int32_t s = q < 27 ? __builtin_amdgcn_readlane(qa, q) : 0;
-      float4 rn = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (q < 27 && s + l < __builtin_amdgcn_readlane(qb, q)) rn = rec[s + l];
-      while (q < 27) {
-        const int32_t b = __builtin_amdgcn_readlane(qb, q);
+      // the position after chunk (q, s): the next chunk of cell q, else the
+      // first of the next cell with records (27: none left)
+      auto next_pos = [&](int &q, int32_t &s) {
+        if (q >= 27) return;
+        s += 64;
+        if (s >= __builtin_amdgcn_readlane(qb, q)) {
+          rest &= ~(1ull << q);
+          q = rest ? __ffsll((unsigned long long)rest) - 1 : 27;
+          s = q < 27 ? __builtin_amdgcn_readlane(qa, q) : 0;
+        }
+      };
+      // unconditional loads, clamped to the chunk's cell (rec[0] when there
+      // is no chunk): no branch around them, so each wait is for the oldest
+      auto load = [&](int q, int32_t s) {
+        const int32_t i =
+            q < 27 ? min(s + l, __builtin_amdgcn_readlane(qb, q) - 1) : 0;
+        const float4 v = rec[i];
+        return make_float3(v.x, v.y, v.z);
+      };
+      // the terms of chunk (q, s) of records r; the cell's sums are reduced
+      // when it is the cell's last chunk
+      auto sum_chunk = [&](int q, int32_t s, const float3 &r, bool cell_end) {
         int ei, ej, ek;
         walk_off(q, 1, &ei, &ej, &ek);
         const int di = ci + ei, dj = cj + ej, dk = ck + ek;
-        const float4 r = rn;
-        const bool valid = s + l < b;
-        int qn = q;
-        int32_t sn = s + 64;
-        if (sn >= b) {
-          rest &= ~(1ull << q);
-          qn = rest ? __ffsll((unsigned long long)rest) - 1 : 27;
-          sn = qn < 27 ? __builtin_amdgcn_readlane(qa, qn) : 0;
-        }
-        if (qn < 27 && sn + l < __builtin_amdgcn_readlane(qb, qn)) rn = rec[sn + l];
+        const bool valid = s + l < __builtin_amdgcn_readlane(qb, q);
         // block-local coordinates of the walked cell (8 divides nx)
         const int32_t lx = (pl.cs[0] + di) & 7, ly = (pl.cs[1] + dj) & 7,
                       lz = (pl.cs[2] + dk) & 7;
@@ -909,7 +914,7 @@ k_centre_bound(SodCfg c, const int32_t *__restrict__ cell_start,
             }
           }
         }
-        if (qn != q || ++nch == 255) {
+        if (cell_end || ++nch == 255) {
           uint64_t u[8];
 #pragma unroll
           for (int g = 0; g < 8; g++) {
@@ -919,14 +924,47 @@ k_centre_bound(SodCfg c, const int32_t *__restrict__ cell_start,
           for (int g = 0; g < 4; g++) ubg[g] = 0;
           cellv += group_reduce8(u, l);
           nch = 0;
-          if (qn != q) {  // the cell is done
+          if (cell_end) {  // the cell is done
             tot += cellv;
             if ((l & 7) == 0) cb[q * 8 + (l >> 3)] = cellv;
             cellv = 0;
           }
         }
-        q = qn;
-        s = sn;
+      };
+      // chunks of 64 records of the cells in turn, three in flight: slots A,
+      // B, C hold consecutive chunks; once a slot's chunk is summed, the slot
+      // requests the chunk after the last one requested (tq, ts).  With one
+      // chunk in flight the kernel took 66.7 ms at 1024^3, with three 65.3.
+      int qA = rest ? __ffsll((unsigned long long)rest) - 1 : 27;
+      int32_t sA = qA < 27 ? __builtin_amdgcn_readlane(qa, qA) : 0;
+      int qB = qA, qC;
+      int32_t sB = sA, sC;
+      next_pos(qB, sB);
+      qC = qB;
+      sC = sB;
+      next_pos(qC, sC);
+      int tq = qC;
+      int32_t ts = sC;
+      float3 rA = load(qA, sA), rB = load(qB, sB), rC = load(qC, sC);
+      for (;;) {
+        if (qA >= 27) break;
+        sum_chunk(qA, sA, rA, qB != qA);
+        next_pos(tq, ts);
+        qA = tq;
+        sA = ts;
+        rA = load(qA, sA);
+        if (qB >= 27) break;
+        sum_chunk(qB, sB, rB, qC != qB);
+        next_pos(tq, ts);
+        qB = tq;
+        sB = ts;
+        rB = load(qB, sB);
+        if (qC >= 27) break;
+        sum_chunk(qC, sC, rC, qA != qC);
+        next_pos(tq, ts);
+        qC = tq;
+        sC = ts;
+        rC = load(qC, sC);
       }
       if ((l & 7) == 0) cub[((int64_t)tcap * 27 + t) * 8 + (l >> 3)] = tot;
       unsigned long long ubmax = (l >> 3) < ngr ? tot : 0ull;
