@@ -281,7 +281,10 @@ __global__ void k_cs_split(const int32_t *__restrict__ O, int dbits,
 // Dense cells (at least DENSE records) are reordered after the sort; those
 // of at most SUB_REG records fit the registers of one workgroup.
 constexpr int DENSE = 64;
-constexpr int SUB_T = 256, SUB_V = 16, SUB_REG = SUB_T * SUB_V;
+// cells of at most SUB_REG records are sorted in registers, SUB_V records
+// per thread: one wave per cell up to SUB_W records, SUB_T threads above
+constexpr int SUB_V = 8, SUB_T = 512, SUB_REG = SUB_T * SUB_V,
+              SUB_W = 64 * SUB_V;
 
 // One workgroup per block: counting sort of its records by local cell, the
 // final records and the block's 512 cell offsets.  A block of at most BS_CAP
@@ -301,8 +304,9 @@ k_block_sort(const int32_t *__restrict__ bstart, int64_t nblk,
              const int32_t *__restrict__ stage_lab,
              int32_t *__restrict__ cell_start, float4 *__restrict__ rec,
              int32_t *__restrict__ perm, int32_t *__restrict__ lab_sorted,
-             int32_t *__restrict__ dlist, int32_t *__restrict__ dbig,
-             int32_t *__restrict__ dcnt, int32_t dcap) {
+             int32_t *__restrict__ dlist, int32_t *__restrict__ dmid,
+             int32_t *__restrict__ dbig, int32_t *__restrict__ dcnt,
+             int32_t dcap) {
   __shared__ int32_t cnt[512];
   __shared__ float4 sr[BS_CAP];
   __shared__ int32_t sl[BS_CAP];
@@ -367,30 +371,37 @@ k_block_sort(const int32_t *__restrict__ bstart, int64_t nblk,
       for (int k = 0; k < 8; k++) { v[k] = cnt[threadIdx.x * 8 + k]; s += v[k]; }
       const int32_t inc = wave_incl_scan(s);
       int32_t e = inc - s;
-      int nr = 0, ng = 0;  // this lane's dense cells, by list
+      int nr = 0, nm = 0, ng = 0;  // this lane's dense cells, by list
       for (int k = 0; k < 8; k++) {
         cnt[threadIdx.x * 8 + k] = e;
         cell_start[b * 512 + threadIdx.x * 8 + k] = a + e;
         e += v[k];
-        nr += v[k] >= DENSE && v[k] <= SUB_REG;
+        nr += v[k] >= DENSE && v[k] <= SUB_W;
+        nm += v[k] > SUB_W && v[k] <= SUB_REG;
         ng += v[k] > SUB_REG;
       }
       // dense cells to their lists: one atomic per block and list
-      const int32_t ir = wave_incl_scan(nr), ig = wave_incl_scan(ng);
-      const int32_t tr = __shfl(ir, 63, 64), tg = __shfl(ig, 63, 64);
-      const int32_t er = ir - nr, eg = ig - ng;
-      int32_t br = 0, bg = 0;
+      const int32_t ir = wave_incl_scan(nr), im = wave_incl_scan(nm),
+                    ig = wave_incl_scan(ng);
+      const int32_t tr = __shfl(ir, 63, 64), tm = __shfl(im, 63, 64),
+                    tg = __shfl(ig, 63, 64);
+      int32_t br = 0, bm = 0, bg = 0;
       if (threadIdx.x == 0) {
         if (tr > 0) br = atomicAdd(&dcnt[0], tr);
         if (tg > 0) bg = atomicAdd(&dcnt[2], tg);
+        if (tm > 0) bm = atomicAdd(&dcnt[4], tm);
       }
-      br = __shfl(br, 0, 64) + er;
-      bg = __shfl(bg, 0, 64) + eg;
+      br = __shfl(br, 0, 64) + ir - nr;
+      bm = __shfl(bm, 0, 64) + im - nm;
+      bg = __shfl(bg, 0, 64) + ig - ng;
       for (int k = 0; k < 8; k++) {
         const int32_t cell = (int32_t)(b * 512 + threadIdx.x * 8 + k);
-        if (v[k] >= DENSE && v[k] <= SUB_REG) {
+        if (v[k] >= DENSE && v[k] <= SUB_W) {
           if (br < dcap) dlist[br] = cell;
           br++;
+        } else if (v[k] > SUB_W && v[k] <= SUB_REG) {
+          if (bm < dcap) dmid[bm] = cell;
+          bm++;
         } else if (v[k] > SUB_REG) {
           if (bg < dcap) dbig[bg] = cell;
           bg++;
@@ -462,7 +473,19 @@ __device__ inline uint32_t morton9(const float4 &r) {
   return spread3(qx) | (spread3(qy) << 1) | (spread3(qz) << 2);
 }
 
-__global__ void __launch_bounds__(SUB_T)
+// Cells of at most T * SUB_V records (BIG false): the records read once into
+// registers (SUB_V per thread) and placed in place; one wave per cell up to
+// SUB_W records, SUB_T threads up to SUB_REG (8 records per thread keep the
+// registers to a quarter of the file: 16 took all of it).  Larger cells (BIG true):
+// SUB_TB threads, staged through the workgroup's slice, each pass issuing a
+// thread's SUB_U records before using them (that launch: 0.88 -> 0.42 ms at
+// 1024^3).  A cell's steps are serial round trips to memory, so the
+// launches over the many small cells are bound by the cells in flight: one
+// 256-thread workgroup per cell of up to 4096 records took 5.7 ms.
+constexpr int SUB_TB = 1024, SUB_U = 4;
+
+template <int T, bool BIG>
+__global__ void __launch_bounds__(T)
 k_subsort(const int32_t *__restrict__ cs, const int32_t *__restrict__ list,
           const int32_t *__restrict__ nlist, int32_t cap,
           int32_t *__restrict__ queue,
@@ -473,26 +496,58 @@ k_subsort(const int32_t *__restrict__ cs, const int32_t *__restrict__ list,
   __shared__ int32_t hist[512];
   __shared__ int32_t item;
   const int32_t nl = min(*nlist, cap);
-  float4 *tr = trec ? trec + (int64_t)blockIdx.x * tcap : nullptr;
-  int32_t *tp = tperm ? tperm + (int64_t)blockIdx.x * tcap : nullptr;
-  int32_t *tl = tlab ? tlab + (int64_t)blockIdx.x * tcap : nullptr;
+  float4 *tr = BIG ? trec + (int64_t)blockIdx.x * tcap : nullptr;
+  int32_t *tp = BIG && tperm ? tperm + (int64_t)blockIdx.x * tcap : nullptr;
+  int32_t *tl = BIG && tlab ? tlab + (int64_t)blockIdx.x * tcap : nullptr;
+  // cells are taken SUB_B at a time: one counter shared by every workgroup
+  // of the launch, hundreds of thousands of small cells
+  constexpr int SUB_B = BIG ? 1 : 16;
+  int32_t it = 0, end = 0;
   for (;;) {
-    if (threadIdx.x == 0) item = atomicAdd(queue, 1);
-    for (int i = threadIdx.x; i < 512; i += SUB_T) hist[i] = 0;
+    if (it == end) {
+      if (threadIdx.x == 0) item = atomicAdd(queue, SUB_B);
+      __syncthreads();
+      it = item;
+      end = it + SUB_B;
+      __syncthreads();
+    }
+    for (int i = threadIdx.x; i < 512; i += T) hist[i] = 0;
     __syncthreads();
-    const int32_t it = item;
     if (it >= nl) break;
-    const int32_t c = list[it];
+    const int32_t c = list[it++];
     const int32_t a = cs[c], n = cs[c + 1] - a;
-    // cells up to SUB_REG records are done in registers (no slice: tcap
-    // is 0 then); larger ones need the staging slice
-    if (trec ? n > tcap : n > SUB_REG) {
+    if (BIG ? n > tcap : n > T * SUB_V) {
       // would not fit: keep the scatter order
       __syncthreads();
       continue;
     }
-    for (int32_t t = threadIdx.x; t < n; t += SUB_T)
-      atomicAdd(&hist[morton9(rec[a + t])], 1);
+    // the register path reads its records here, once
+    float4 rr[BIG ? 1 : SUB_V];
+    int32_t pp[BIG ? 1 : SUB_V], ll[BIG ? 1 : SUB_V];
+    if constexpr (BIG) {
+      for (int32_t t0 = threadIdx.x; t0 < n; t0 += T * SUB_U) {
+        float4 r[SUB_U];
+#pragma unroll
+        for (int k = 0; k < SUB_U; k++)
+          if (t0 + k * T < n) r[k] = rec[a + t0 + k * T];
+#pragma unroll
+        for (int k = 0; k < SUB_U; k++)
+          if (t0 + k * T < n) atomicAdd(&hist[morton9(r[k])], 1);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < SUB_V; k++) {
+        const int32_t t = threadIdx.x + k * T;
+        if (t < n) {
+          rr[k] = rec[a + t];
+          pp[k] = perm ? perm[a + t] : 0;
+          ll[k] = lab ? lab[a + t] : 0;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < SUB_V; k++)
+        if (threadIdx.x + k * T < n) atomicAdd(&hist[morton9(rr[k])], 1);
+    }
     __syncthreads();
     if (threadIdx.x < 64) {  // exclusive scan of 512 counters by one wave
       int32_t v[8], s = 0;
@@ -502,24 +557,16 @@ k_subsort(const int32_t *__restrict__ cs, const int32_t *__restrict__ list,
       for (int k = 0; k < 8; k++) { hist[threadIdx.x * 8 + k] = e; e += v[k]; }
     }
     __syncthreads();
-    if (n <= SUB_REG) {
-      // in place: everything is read into registers before any write
-      float4 rr[SUB_V];
-      int32_t pp[SUB_V], ll[SUB_V], dd[SUB_V];
+    if constexpr (!BIG) {
+      // in place: everything was read into registers before any write
+      int32_t dd[SUB_V];
 #pragma unroll
-      for (int k = 0; k < SUB_V; k++) {
-        const int32_t t = threadIdx.x + k * SUB_T;
-        if (t < n) {
-          rr[k] = rec[a + t];
-          pp[k] = perm ? perm[a + t] : 0;
-          ll[k] = lab ? lab[a + t] : 0;
-          dd[k] = atomicAdd(&hist[morton9(rr[k])], 1);
-        }
-      }
+      for (int k = 0; k < SUB_V; k++)
+        if (threadIdx.x + k * T < n) dd[k] = atomicAdd(&hist[morton9(rr[k])], 1);
       __syncthreads();
 #pragma unroll
       for (int k = 0; k < SUB_V; k++) {
-        const int32_t t = threadIdx.x + k * SUB_T;
+        const int32_t t = threadIdx.x + k * T;
         if (t < n) {
           rec[a + dd[k]] = rr[k];
           if (perm) perm[a + dd[k]] = pp[k];
@@ -527,19 +574,51 @@ k_subsort(const int32_t *__restrict__ cs, const int32_t *__restrict__ list,
         }
       }
     } else {
-      for (int32_t t = threadIdx.x; t < n; t += SUB_T) {
-        const float4 r = rec[a + t];
-        const int32_t d = atomicAdd(&hist[morton9(r)], 1);
-        tr[d] = r;
-        if (perm) tp[d] = perm[a + t];
-        if (lab) tl[d] = lab[a + t];
+      for (int32_t t0 = threadIdx.x; t0 < n; t0 += T * SUB_U) {
+        float4 r[SUB_U];
+        int32_t pv[SUB_U], lv[SUB_U];
+#pragma unroll
+        for (int k = 0; k < SUB_U; k++) {
+          const int32_t t = t0 + k * T;
+          if (t < n) {
+            r[k] = rec[a + t];
+            pv[k] = perm ? perm[a + t] : 0;
+            lv[k] = lab ? lab[a + t] : 0;
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < SUB_U; k++) {
+          if (t0 + k * T < n) {
+            const int32_t d = atomicAdd(&hist[morton9(r[k])], 1);
+            tr[d] = r[k];
+            if (perm) tp[d] = pv[k];
+            if (lab) tl[d] = lv[k];
+          }
+        }
       }
       __threadfence_block();
       __syncthreads();
-      for (int32_t t = threadIdx.x; t < n; t += SUB_T) {
-        rec[a + t] = tr[t];
-        if (perm) perm[a + t] = tp[t];
-        if (lab) lab[a + t] = tl[t];
+      for (int32_t t0 = threadIdx.x; t0 < n; t0 += T * SUB_U) {
+        float4 r[SUB_U];
+        int32_t pv[SUB_U], lv[SUB_U];
+#pragma unroll
+        for (int k = 0; k < SUB_U; k++) {
+          const int32_t t = t0 + k * T;
+          if (t < n) {
+            r[k] = tr[t];
+            pv[k] = perm ? tp[t] : 0;
+            lv[k] = lab ? tl[t] : 0;
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < SUB_U; k++) {
+          const int32_t t = t0 + k * T;
+          if (t < n) {
+            rec[a + t] = r[k];
+            if (perm) perm[a + t] = pv[k];
+            if (lab) lab[a + t] = lv[k];
+          }
+        }
       }
     }
     __syncthreads();
@@ -633,28 +712,35 @@ extern "C" int psod_cell_sort(psod_ctx *ctx, int64_t npart, const float *d_pos,
   const int32_t *bstart = bs;  // nblk + 1 block starts
   const float4 *stage = recb[cur];
   const int32_t *slab = labb[cur];
-  // lists of the dense cells, filled by the block sort: ctl[0] and ctl[2]
-  // count them, ctl[1] and ctl[3] are the subsort's queues (at most
+  // lists of the dense cells by size, filled by the block sort: ctl[0],
+  // ctl[4] and ctl[2] count them (up to SUB_W, up to SUB_REG, larger
+  // records), ctl[1], ctl[5] and ctl[3] are the subsort's queues (at most
   // npart / DENSE cells are dense)
   const int32_t cap = (int32_t)std::min<int64_t>(ncell, npart / DENSE + 32);
   int32_t *list = (int32_t *)psod_scratch(ctx, S_HIST,
-                                          (size_t)(2 * (int64_t)cap + 64) * 4);
+                                          (size_t)(3 * (int64_t)cap + 96) * 4);
   int32_t *ctl = (int32_t *)psod_scratch(ctx, S_MISC, 64);
   if (!list || !ctl) return PSOD_E_NOMEM;
   int32_t *big = list + cap + 32;
-  PSOD_TRY(psod_fill_i32(ctx, ctl, 4, 0));
+  int32_t *mid = big + cap + 32;
+  PSOD_TRY(psod_fill_i32(ctx, ctl, 6, 0));
   k_block_sort<<<grid_for(nblk, 1, (int64_t)ctx->num_cus * 32), 256, 0, st>>>(
       bstart, nblk, stage, slab, d_cell_start, (float4 *)d_rec, d_perm,
-      d_lab_sorted, list, big, ctl, cap);
+      d_lab_sorted, list, mid, big, ctl, cap);
   PSOD_LAUNCH_CHECK();
   k_set_last<<<1, 64, 0, st>>>(d_cell_start, ncell, (int32_t)npart);
   PSOD_LAUNCH_CHECK();
   if (npart > 0) {
     // Morton order of the 1/8-cell sub-cells inside dense cells
-    // cells that fit the registers: many workgroups, no staging slices;
-    // larger cells: one workgroup per CU, each with a staging slice
-    k_subsort<<<(unsigned)ctx->num_cus * 8, SUB_T, 0, st>>>(
+    // cells that fit the registers: one wave per cell up to SUB_W records,
+    // SUB_T threads up to SUB_REG, no staging slices; larger cells: one
+    // workgroup per CU, each with a staging slice
+    k_subsort<64, false><<<(unsigned)ctx->num_cus * 12, 64, 0, st>>>(
         d_cell_start, list, ctl, cap, ctl + 1, (float4 *)d_rec, d_perm,
+        d_lab_sorted, nullptr, nullptr, nullptr, 0);
+    PSOD_LAUNCH_CHECK();
+    k_subsort<SUB_T, false><<<(unsigned)ctx->num_cus * 2, SUB_T, 0, st>>>(
+        d_cell_start, mid, ctl + 4, cap, ctl + 5, (float4 *)d_rec, d_perm,
         d_lab_sorted, nullptr, nullptr, nullptr, 0);
     PSOD_LAUNCH_CHECK();
     // cells above SUB_REG records: at most npart / SUB_REG of them, each
@@ -664,7 +750,7 @@ extern "C" int psod_cell_sort(psod_ctx *ctx, int64_t npart, const float *d_pos,
                                                       npart / SUB_REG + 1);
     char *tmp = (char *)psod_scratch(ctx, S_KEYS, (size_t)nsub * tcap * 24);
     if (!tmp) return PSOD_E_NOMEM;
-    k_subsort<<<nsub, SUB_T, 0, st>>>(
+    k_subsort<SUB_TB, true><<<nsub, SUB_TB, 0, st>>>(
         d_cell_start, big, ctl + 2, cap, ctl + 3, (float4 *)d_rec, d_perm,
         d_lab_sorted, (float4 *)tmp, (int32_t *)(tmp + (size_t)nsub * tcap * 16),
         (int32_t *)(tmp + (size_t)nsub * tcap * 20), tcap);
