@@ -289,13 +289,66 @@ constexpr int SUB_V = 8, SUB_T = 512, SUB_REG = SUB_T * SUB_V,
 // One workgroup per block: counting sort of its records by local cell, the
 // final records and the block's 512 cell offsets.  A block of at most BS_CAP
 // records is read once into registers, placed in LDS in cell order and
-// written out as one contiguous run; a larger one is read twice (count, then
-// place) and written record by record.
+// written out as one contiguous run; a larger one is listed for
+// k_block_sort_big, which reads it twice (count, then place) and writes it
+// record by record.
 constexpr int BS_CAP = 1024;  // 22 KB of LDS: 6 workgroups per CU (by registers)
 constexpr int BS_U = BS_CAP / 256;
 
 __device__ inline int32_t local_cell(const float4 &r) {
   return ((int32_t)r.z * 8 + (int32_t)r.y) * 8 + (int32_t)r.x;
+}
+
+// The block's cell offsets from its 512 counts, run by the first wave: cnt
+// becomes the exclusive scan, cell_start[b * 512 + i] = a + scan, and the
+// dense cells go to their lists by size (one atomic per block and list).
+__device__ inline void block_cells(int32_t *cnt, int64_t b, int32_t a,
+                                   int32_t *__restrict__ cell_start,
+                                   int32_t *__restrict__ dlist,
+                                   int32_t *__restrict__ dmid,
+                                   int32_t *__restrict__ dbig,
+                                   int32_t *__restrict__ dcnt, int32_t dcap) {
+  // exclusive scan of 512 counters by one wave
+  int32_t v[8], s = 0;
+  for (int k = 0; k < 8; k++) { v[k] = cnt[threadIdx.x * 8 + k]; s += v[k]; }
+  const int32_t inc = wave_incl_scan(s);
+  int32_t e = inc - s;
+  int nr = 0, nm = 0, ng = 0;  // this lane's dense cells, by list
+  for (int k = 0; k < 8; k++) {
+    cnt[threadIdx.x * 8 + k] = e;
+    cell_start[b * 512 + threadIdx.x * 8 + k] = a + e;
+    e += v[k];
+    nr += v[k] >= DENSE && v[k] <= SUB_W;
+    nm += v[k] > SUB_W && v[k] <= SUB_REG;
+    ng += v[k] > SUB_REG;
+  }
+  // dense cells to their lists: one atomic per block and list
+  const int32_t ir = wave_incl_scan(nr), im = wave_incl_scan(nm),
+                ig = wave_incl_scan(ng);
+  const int32_t tr = __shfl(ir, 63, 64), tm = __shfl(im, 63, 64),
+                tg = __shfl(ig, 63, 64);
+  int32_t br = 0, bm = 0, bg = 0;
+  if (threadIdx.x == 0) {
+    if (tr > 0) br = atomicAdd(&dcnt[0], tr);
+    if (tg > 0) bg = atomicAdd(&dcnt[2], tg);
+    if (tm > 0) bm = atomicAdd(&dcnt[4], tm);
+  }
+  br = __shfl(br, 0, 64) + ir - nr;
+  bm = __shfl(bm, 0, 64) + im - nm;
+  bg = __shfl(bg, 0, 64) + ig - ng;
+  for (int k = 0; k < 8; k++) {
+    const int32_t cell = (int32_t)(b * 512 + threadIdx.x * 8 + k);
+    if (v[k] >= DENSE && v[k] <= SUB_W) {
+      if (br < dcap) dlist[br] = cell;
+      br++;
+    } else if (v[k] > SUB_W && v[k] <= SUB_REG) {
+      if (bm < dcap) dmid[bm] = cell;
+      bm++;
+    } else if (v[k] > SUB_REG) {
+      if (bg < dcap) dbig[bg] = cell;
+      bg++;
+    }
+  }
 }
 
 __global__ void __launch_bounds__(256)
@@ -306,7 +359,7 @@ k_block_sort(const int32_t *__restrict__ bstart, int64_t nblk,
              int32_t *__restrict__ perm, int32_t *__restrict__ lab_sorted,
              int32_t *__restrict__ dlist, int32_t *__restrict__ dmid,
              int32_t *__restrict__ dbig, int32_t *__restrict__ dcnt,
-             int32_t dcap) {
+             int32_t dcap, int32_t *__restrict__ bigblk) {
   __shared__ int32_t cnt[512];
   __shared__ float4 sr[BS_CAP];
   __shared__ int32_t sl[BS_CAP];
@@ -351,65 +404,22 @@ k_block_sort(const int32_t *__restrict__ bstart, int64_t nblk,
       a2 = bstart[b + 2 * G];
       n2 = bstart[b + 2 * G + 1] - a2;
     }
-    for (int i = threadIdx.x; i < 512; i += 256) cnt[i] = 0;
-    __syncthreads();
-    const bool staged = n <= BS_CAP;
-    int32_t dd[BS_U];
-    if (staged) {
+    if (n > BS_CAP) {
+      // placed by k_block_sort_big
+      if (threadIdx.x == 0) bigblk[atomicAdd(&dcnt[6], 1)] = (int32_t)b;
+    } else {
+      for (int i = threadIdx.x; i < 512; i += 256) cnt[i] = 0;
+      __syncthreads();
+      int32_t dd[BS_U];
 #pragma unroll
       for (int k = 0; k < BS_U; k++) {
         const int32_t t = threadIdx.x + 256 * k;
         if (t < n) dd[k] = atomicAdd(&cnt[local_cell(rr[k])], 1);
       }
-    } else {
-      for (int32_t t = threadIdx.x; t < n; t += 256)
-        atomicAdd(&cnt[local_cell(stage[a + t])], 1);
-    }
-    __syncthreads();
-    if (threadIdx.x < 64) {  // exclusive scan of 512 counters by one wave
-      int32_t v[8], s = 0;
-      for (int k = 0; k < 8; k++) { v[k] = cnt[threadIdx.x * 8 + k]; s += v[k]; }
-      const int32_t inc = wave_incl_scan(s);
-      int32_t e = inc - s;
-      int nr = 0, nm = 0, ng = 0;  // this lane's dense cells, by list
-      for (int k = 0; k < 8; k++) {
-        cnt[threadIdx.x * 8 + k] = e;
-        cell_start[b * 512 + threadIdx.x * 8 + k] = a + e;
-        e += v[k];
-        nr += v[k] >= DENSE && v[k] <= SUB_W;
-        nm += v[k] > SUB_W && v[k] <= SUB_REG;
-        ng += v[k] > SUB_REG;
-      }
-      // dense cells to their lists: one atomic per block and list
-      const int32_t ir = wave_incl_scan(nr), im = wave_incl_scan(nm),
-                    ig = wave_incl_scan(ng);
-      const int32_t tr = __shfl(ir, 63, 64), tm = __shfl(im, 63, 64),
-                    tg = __shfl(ig, 63, 64);
-      int32_t br = 0, bm = 0, bg = 0;
-      if (threadIdx.x == 0) {
-        if (tr > 0) br = atomicAdd(&dcnt[0], tr);
-        if (tg > 0) bg = atomicAdd(&dcnt[2], tg);
-        if (tm > 0) bm = atomicAdd(&dcnt[4], tm);
-      }
-      br = __shfl(br, 0, 64) + ir - nr;
-      bm = __shfl(bm, 0, 64) + im - nm;
-      bg = __shfl(bg, 0, 64) + ig - ng;
-      for (int k = 0; k < 8; k++) {
-        const int32_t cell = (int32_t)(b * 512 + threadIdx.x * 8 + k);
-        if (v[k] >= DENSE && v[k] <= SUB_W) {
-          if (br < dcap) dlist[br] = cell;
-          br++;
-        } else if (v[k] > SUB_W && v[k] <= SUB_REG) {
-          if (bm < dcap) dmid[bm] = cell;
-          bm++;
-        } else if (v[k] > SUB_REG) {
-          if (bg < dcap) dbig[bg] = cell;
-          bg++;
-        }
-      }
-    }
-    __syncthreads();
-    if (staged) {
+      __syncthreads();
+      if (threadIdx.x < 64)
+        block_cells(cnt, b, a, cell_start, dlist, dmid, dbig, dcnt, dcap);
+      __syncthreads();
 #pragma unroll
       for (int k = 0; k < BS_U; k++) {
         const int32_t t = threadIdx.x + 256 * k;
@@ -426,16 +436,8 @@ k_block_sort(const int32_t *__restrict__ bstart, int64_t nblk,
         if (perm) perm[a + t] = __float_as_int(r.w);
         if (lab_sorted) lab_sorted[a + t] = sl[t];
       }
-    } else {
-      for (int32_t t = threadIdx.x; t < n; t += 256) {
-        const float4 r = stage[a + t];
-        const int32_t d = a + atomicAdd(&cnt[local_cell(r)], 1);
-        rec[d] = r;
-        if (perm) perm[d] = __float_as_int(r.w);
-        if (lab_sorted) lab_sorted[d] = stage_lab[a + t];
-      }
+      __syncthreads();
     }
-    __syncthreads();
     a = an;
     n = nn;
     an = a2;
@@ -445,6 +447,71 @@ k_block_sort(const int32_t *__restrict__ bstart, int64_t nblk,
       rr[k] = rn[k];
       ll[k] = ln[k];
     }
+  }
+}
+
+// Blocks of more than BS_CAP records, one BB_T-thread workgroup each: the
+// records are counted per cell, then placed from HBM, each pass issuing a
+// thread's BB_U records before using them.  Placed by the 256-thread block
+// sort with one record in flight per thread, the zoom catalogue's central
+// blocks (up to ~10^5 records each) held that launch for 30 ms.
+constexpr int BB_T = 1024, BB_U = 4;
+
+__global__ void __launch_bounds__(BB_T)
+k_block_sort_big(const int32_t *__restrict__ bstart,
+                 const float4 *__restrict__ stage,
+                 const int32_t *__restrict__ stage_lab,
+                 int32_t *__restrict__ cell_start, float4 *__restrict__ rec,
+                 int32_t *__restrict__ perm, int32_t *__restrict__ lab_sorted,
+                 int32_t *__restrict__ dlist, int32_t *__restrict__ dmid,
+                 int32_t *__restrict__ dbig, int32_t *__restrict__ dcnt,
+                 int32_t dcap, const int32_t *__restrict__ bigblk) {
+  __shared__ int32_t cnt[512];
+  __shared__ int32_t item;
+  const int32_t nbig = dcnt[6];
+  for (;;) {
+    if (threadIdx.x == 0) item = atomicAdd(&dcnt[7], 1);
+    for (int i = threadIdx.x; i < 512; i += BB_T) cnt[i] = 0;
+    __syncthreads();
+    const int32_t it = item;
+    if (it >= nbig) break;
+    const int64_t b = bigblk[it];
+    const int32_t a = bstart[b], n = bstart[b + 1] - a;
+    for (int32_t t0 = threadIdx.x; t0 < n; t0 += BB_T * BB_U) {
+      float4 r[BB_U];
+#pragma unroll
+      for (int k = 0; k < BB_U; k++)
+        if (t0 + k * BB_T < n) r[k] = stage[a + t0 + k * BB_T];
+#pragma unroll
+      for (int k = 0; k < BB_U; k++)
+        if (t0 + k * BB_T < n) atomicAdd(&cnt[local_cell(r[k])], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < 64)
+      block_cells(cnt, b, a, cell_start, dlist, dmid, dbig, dcnt, dcap);
+    __syncthreads();
+    for (int32_t t0 = threadIdx.x; t0 < n; t0 += BB_T * BB_U) {
+      float4 r[BB_U];
+      int32_t lv[BB_U];
+#pragma unroll
+      for (int k = 0; k < BB_U; k++) {
+        const int32_t t = t0 + k * BB_T;
+        if (t < n) {
+          r[k] = stage[a + t];
+          lv[k] = stage_lab ? stage_lab[a + t] : 0;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < BB_U; k++) {
+        if (t0 + k * BB_T < n) {
+          const int32_t d = a + atomicAdd(&cnt[local_cell(r[k])], 1);
+          rec[d] = r[k];
+          if (perm) perm[d] = __float_as_int(r[k].w);
+          if (lab_sorted) lab_sorted[d] = lv[k];
+        }
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -717,16 +784,23 @@ extern "C" int psod_cell_sort(psod_ctx *ctx, int64_t npart, const float *d_pos,
   // records), ctl[1], ctl[5] and ctl[3] are the subsort's queues (at most
   // npart / DENSE cells are dense)
   const int32_t cap = (int32_t)std::min<int64_t>(ncell, npart / DENSE + 32);
-  int32_t *list = (int32_t *)psod_scratch(ctx, S_HIST,
-                                          (size_t)(3 * (int64_t)cap + 96) * 4);
+  // blocks of more than BS_CAP records: listed by the block sort (count
+  // in ctl[6], queue ctl[7]) and placed by k_block_sort_big
+  int32_t *list = (int32_t *)psod_scratch(
+      ctx, S_HIST, (size_t)(3 * (int64_t)cap + 96 + nblk + 32) * 4);
   int32_t *ctl = (int32_t *)psod_scratch(ctx, S_MISC, 64);
   if (!list || !ctl) return PSOD_E_NOMEM;
   int32_t *big = list + cap + 32;
   int32_t *mid = big + cap + 32;
-  PSOD_TRY(psod_fill_i32(ctx, ctl, 6, 0));
+  int32_t *bigblk = mid + cap + 32;
+  PSOD_TRY(psod_fill_i32(ctx, ctl, 8, 0));
   k_block_sort<<<grid_for(nblk, 1, (int64_t)ctx->num_cus * 32), 256, 0, st>>>(
       bstart, nblk, stage, slab, d_cell_start, (float4 *)d_rec, d_perm,
-      d_lab_sorted, list, mid, big, ctl, cap);
+      d_lab_sorted, list, mid, big, ctl, cap, bigblk);
+  PSOD_LAUNCH_CHECK();
+  k_block_sort_big<<<(unsigned)ctx->num_cus, BB_T, 0, st>>>(
+      bstart, stage, slab, d_cell_start, (float4 *)d_rec, d_perm,
+      d_lab_sorted, list, mid, big, ctl, cap, bigblk);
   PSOD_LAUNCH_CHECK();
   k_set_last<<<1, 64, 0, st>>>(d_cell_start, ncell, (int32_t)npart);
   PSOD_LAUNCH_CHECK();
