@@ -490,6 +490,7 @@ def main():
         "roofline": roofline,
         "centre_search": centre,
         "sod_paths": dict(wave=cs["wave"], workgroup=cs["workgroup"],
+                          workgroup_sorted_in_lds=cs["light"],
                           rerun=cs["retried"], giants_staged=cs["giants"],
                           giant_cycles=cs["giant_cycles"]),
         "plain_fp64": plain,

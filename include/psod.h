@@ -54,7 +54,7 @@ int psod_synchronize(psod_ctx *ctx);
 void *psod_stream(psod_ctx *ctx);
 /* bytes of device scratch the context currently holds                       */
 int64_t psod_scratch_bytes(psod_ctx *ctx);
-/* counts of the last psod_sod_fof call, first min(n, 12) of: out[0] halos
+/* counts of the last psod_sod_fof call, first min(n, 13) of: out[0] halos
  * grown by one wave, out[1] halos grown by one workgroup, out[2] of these
  * re-run with larger collection slices, out[3] the keys per slice of that
  * re-run (0 if none), out[4] centre-search units summed to the end (pilot
@@ -63,7 +63,8 @@ int64_t psod_scratch_bytes(psod_ctx *ctx);
  * tasks in all, out[8] centre-search time in ns, out[9] items abandoned
  * part way (exact sums plus the bound of the remaining neighbour cells
  * below the best S), out[10] giant halos grown by the staged launches,
- * out[11] their cycles (a cycle per reach tried).
+ * out[11] their cycles (a cycle per reach tried), out[12] halos of the
+ * workgroup path whose growth cube was sorted whole in LDS.
  * No reference counterpart: diagnostics of the execution paths. */
 int psod_sod_stats(psod_ctx *ctx, int64_t *out, int32_t n);
 

@@ -325,13 +325,15 @@ class Context:
         evaluated exactly (pilot tasks and groups of 8 candidates), units
         skipped by their bound, groups abandoned part-way, pair tests and
         the search's time; giant halos grown by the staged launches and
-        their cycles."""
-        a = (C.c_int64 * 12)()
-        _lib.check(_L.psod_sod_stats(self.h, a, 12))
+        their cycles; workgroup halos whose growth cube was sorted whole in
+        LDS."""
+        a = (C.c_int64 * 13)()
+        _lib.check(_L.psod_sod_stats(self.h, a, 13))
         return dict(wave=a[0], workgroup=a[1], retried=a[2], retry_slice=a[3],
                     centre_tasks=a[4], centre_skipped=a[5], pair_tests=a[6],
                     centre_tasks_all=a[7], centre_ms=a[8] * 1e-6,
-                    centre_aborted=a[9], giants=a[10], giant_cycles=a[11])
+                    centre_aborted=a[9], giants=a[10], giant_cycles=a[11],
+                    light=a[12])
 
     # ---- host drop-ins ---------------------------------------------------
     def host_cic(self, pos, ng, mass=None, xxmin=0.0):

@@ -119,7 +119,7 @@ extern "C" int64_t psod_scratch_bytes(psod_ctx *c) {
 
 extern "C" int psod_sod_stats(psod_ctx *c, int64_t *out, int32_t n) {
   if (!c || !out || n < 0) return psod_set_error(PSOD_E_ARG, "psod_sod_stats: bad arguments");
-  for (int i = 0; i < n && i < 12; i++) out[i] = c->sod_stat[i];
+  for (int i = 0; i < n && i < 13; i++) out[i] = c->sod_stat[i];
   return PSOD_OK;
 }
 

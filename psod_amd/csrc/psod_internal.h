@@ -98,7 +98,7 @@ struct psod_ctx {
   void *slot[S_NSLOT];
   size_t cap[S_NSLOT];
   int num_cus;
-  int64_t sod_stat[12];  // see psod_sod_stats
+  int64_t sod_stat[13];  // see psod_sod_stats
   hipEvent_t ev[2];     // brackets the centre search (sod_stat[8])
   // second stream (default priority) for work that overlaps the main one,
   // and the events that fork to it and join from it

@@ -26,6 +26,7 @@
 #include "sod_common.h"
 
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 int psod_fill_i32(psod_ctx *c, int32_t *p, int64_t n, int32_t v);
@@ -35,6 +36,9 @@ constexpr int CAP5 = 1024;   // small path: particles in the 5^3 cells
 constexpr int CAPG = 2048;   // small path: particles in the growth cube
 constexpr int WMAX_S = 4;    // small path: largest reach
 constexpr int NCUBE_S = (2 * WMAX_S + 1) * (2 * WMAX_S + 1) * (2 * WMAX_S + 1);
+// large path: growth cubes of at most this many particles are sorted whole
+// in LDS (PSOD_LIGHT_CAP lowers it; 0 sends every halo through the bins)
+constexpr int32_t LIGHT_CAP = 8192;
 
 __device__ inline int32_t ldcg(const int32_t *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -67,6 +71,11 @@ SodCfg make_cfg(const psod_sod_params *p, int64_t npart) {
   c.tol_abs = p->tol_abs;
   c.nbar = (double)npart / ((double)p->nx * p->nx * p->nx);
   c.giant_min = INT64_MAX;  // no staged growth unless the pass sets it
+  c.light_cap = LIGHT_CAP;
+  if (const char *e = getenv("PSOD_LIGHT_CAP")) {
+    const long v = atol(e);
+    if (v >= 0 && v < LIGHT_CAP) c.light_cap = (int32_t)v;
+  }
   return c;
 }
 
@@ -199,21 +208,38 @@ __device__ inline void wave_argmax(uint64_t &acc, uint64_t &key) {
   }
 }
 
-// bitonic sort of n (power of two) u64 keys in LDS by one workgroup
-__device__ inline void lds_bitonic(uint64_t *a, int n) {
-  for (int k = 2; k <= n; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        const int p = i ^ j;
-        if (p > i) {
-          const uint64_t x = a[i], y = a[p];
-          const bool up = (i & k) == 0;
-          if ((x > y) == up) { a[i] = y; a[p] = x; }
+// Bitonic sort of np (a power of two) u64 keys in LDS by the NT threads of
+// the workgroup (NT a multiple of 64): thread t orders the pairs t, t + NT,
+// ...; pair t of step j is (i, i + j), i being t with a zero bit inserted
+// at j.  For j <= 32 a wave's pairs lie in its own blocks of 128 keys
+// (128 w + 2 NT m), so those steps need a wave barrier only; a workgroup
+// barrier comes before and after every step of larger j, and at the end.
+// (The form that ran every key index and left the upper half idle took
+// twice the iterations.)
+template <int NT>
+__device__ inline void lds_bitonic(uint64_t *K, int32_t np) {
+  const int32_t half = np >> 1;
+  for (int32_t k = 2; k <= np; k <<= 1) {
+    for (int32_t j = k >> 1; j > 0; j >>= 1) {
+      for (int32_t t = threadIdx.x; t < half; t += NT) {
+        const int32_t i = ((t & ~(j - 1)) << 1) | (t & (j - 1)), p = i | j;
+        const uint64_t x = K[i], y = K[p];
+        if ((x > y) == ((i & k) == 0)) {
+          K[i] = y;
+          K[p] = x;
         }
       }
-      __syncthreads();
+      const int32_t jn = j > 1 ? j >> 1 : k;  // the next step's j
+      if (NT > 64 && (j > 32 || jn > 32)) {
+        __syncthreads();
+      } else {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
     }
   }
+  __syncthreads();
 }
 
 __device__ inline int next_pow2(int n) {
@@ -461,7 +487,7 @@ __device__ void small_growth_and_output(const SodCfg &c, SmallLds &S,
     const int npad = next_pow2(ncube < 64 ? 64 : ncube);
     for (int32_t j = ncube + l; j < npad; j += 64) S.b.key[j] = ~0ull;
     __syncthreads();
-    lds_bitonic(S.b.key, npad);
+    lds_bitonic<64>(S.b.key, npad);
     // ---- first crossing of every threshold -----------------------------
     bool complete = true;
     int64_t kst[PSOD_MAXDELTA];
@@ -1867,7 +1893,8 @@ __device__ inline void bin_band(int32_t b0, int32_t b1, int32_t nb,
 // Calls f(d2, idx, rec, di, dj, dk, lc) for the particles of the cells at
 // offsets [-W, W]^3 from the reference cell, skipping the row segments that
 // cannot hold a particle with d2 in [lo2, hi2] (f sees all the others).
-// Segments q0 <= q < q1 only.
+// Segments q0 <= q < q1 only.  An f that takes an eighth argument also gets
+// the particle's ordinal among those walked (0, 1, ... in segment order).
 template <typename F>
 __device__ inline void cube_walk(const SodCfg &c, GrowShared &G,
                                  const int32_t *cell_start, const float4 *rec,
@@ -1876,6 +1903,7 @@ __device__ inline void cube_walk(const SodCfg &c, GrowShared &G,
                                  int64_t q1 = INT64_MAX) {
   const CubeSegs g = cube_segs(c, cr, W);
   const int64_t qe = min(q1, g.nseg);
+  int64_t done = 0;  // particles of the rounds before this one
   for (int64_t base = q0; base < qe; base += GL_T) {
     const int64_t q = base + threadIdx.x;
     int32_t cnt = 0, st = 0, sdi = 0, spk = 0;
@@ -1938,9 +1966,15 @@ __device__ inline void cube_walk(const SodCfg &c, GrowShared &G,
         const float dy = rel1(r.y, lc[1], dj, cr.f[1]);
         const float dz = rel1(r.z, lc[2], dk, cr.f[2]);
         const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-        f(d2, __float_as_int(r.w), r, di, dj, dk, lc);
+        if constexpr (std::is_invocable_v<F &, float, int32_t, const float4 &,
+                                          int, int, int, const int32_t *,
+                                          int64_t>)
+          f(d2, __float_as_int(r.w), r, di, dj, dk, lc, done + j0 + u * GL_T);
+        else
+          f(d2, __float_as_int(r.w), r, di, dj, dk, lc);
       }
     }
+    done += all;
     __syncthreads();
   }
 }
@@ -1984,7 +2018,7 @@ __device__ void block_sort_pool(uint64_t *pool, int32_t n, GrowShared &G) {
     for (int i = threadIdx.x; i < np; i += GL_T)
       G.skey[i] = i < n ? pool[i] : ~0ull;
     __syncthreads();
-    lds_bitonic(G.skey, np);
+    lds_bitonic<GL_T>(G.skey, np);
     for (int i = threadIdx.x; i < n; i += GL_T) pool[i] = G.skey[i];
     __threadfence_block();
     __syncthreads();
@@ -2703,6 +2737,143 @@ __device__ inline void grow_close(const SodCfg &c, int32_t *counters,
   }
 }
 
+// Growth of a halo whose cube of reach W holds n <= LIGHT_CAP particles:
+// the keys (d2, index) of the whole cube are sorted in LDS (K), and the
+// crossings, the second threshold, the profile, the boundary count and the
+// centre of mass are read off the sorted keys, as the one-wave path does
+// for its halos.  Rank k (1-based) is key K[k - 1]: the ranks and radii of
+// the binned steps, so the outputs are theirs.  Only the reach that closes
+// the halo (w, ngather) can differ: the binned steps also double it when no
+// bin surely holds a crossing.  Returns 0 with res and the profile written,
+// else the reach to try next.  A halo whose cube held a few thousand
+// particles took 150-190 us through the bins and the three walks.
+static_assert(LIGHT_CAP <= HIST_LDS / 2, "light keys live in the bins' LDS");
+
+__device__ __forceinline__ int32_t
+grow_light(const SodCfg &c, GrowShared &G, uint64_t *K,
+           const int32_t *cell_start, const float4 *rec, const float *pos,
+           const float *vel, const CellRef &cr, int32_t W, int32_t n,
+           const double *xcs, int64_t h, const psod_halo_out &out,
+           HaloResult &res) {
+  cube_walk(c, G, cell_start, rec, cr, W, -1.0f, INFINITY,
+            [&](float d2, int32_t idx, const float4 &, int, int, int,
+                const int32_t *, int64_t j) { K[j] = d2key(d2, idx); });
+  const int32_t np = next_pow2(n < 2 ? 2 : n);
+  for (int32_t j = n + threadIdx.x; j < np; j += GL_T) K[j] = ~0ull;
+  __syncthreads();
+  lds_bitonic<GL_T>(K, np);
+  // The ranks, the reach test and the growth of the reach are those of the
+  // binned steps: only particles with d2 < W^2 are ranked (they sort before
+  // the others), a threshold not crossed among them doubles the reach, and
+  // a profile reaching beyond the cube extends it.
+  const double Wd = (double)W;
+  const float W2 = (float)W * (float)W;
+  int64_t kst[PSOD_MAXDELTA];
+  for (int di = 0; di < c.ndelta; di++) {
+    // first rank k >= nmin_count with od_k <= delta, unless a particle of
+    // d2 >= W^2 comes first
+    int32_t fk = INT32_MAX;
+    for (int32_t base = c.nmin_count; base <= n && fk == INT32_MAX;
+         base += GL_T) {
+      const int32_t k = base + (int32_t)threadIdx.x;
+      int32_t m = INT32_MAX;
+      if (k <= n) {
+        const float d2 = key_d2(K[k - 1]);
+        if (!(d2 < W2) || od_k(k, c.mpart, (double)sqrtf(d2)) <= c.delta[di])
+          m = k;
+      }
+      fk = block_min_i32(m, G);
+    }
+    if (fk == INT32_MAX || !(key_d2(K[fk - 1]) < W2)) {
+      __syncthreads();
+      return 2 * W;
+    }
+    kst[di] = fk;
+  }
+  for (int di = 0; di < c.ndelta; di++) {
+    const int64_t ks = kst[di];
+    finish_delta(c, di, ks, ref_dist(pos, key_idx(K[ks - 2]), xcs, c.nx),
+                 ref_dist(pos, key_idx(K[ks - 1]), xcs, c.nx), res);
+  }
+  const double R0 = res.radius[0];
+  if (c.rmax_frac * R0 > Wd) {
+    __syncthreads();
+    return max(2 * W, (int32_t)ceil(c.rmax_frac * R0) + 1);
+  }
+  // second threshold: the last rank k in [2, kst[0]] above delta2
+  if (c.delta2 > 0.0) {
+    int32_t best = 0;
+    for (int32_t top = (int32_t)kst[0]; top >= 2 && best == 0; top -= GL_T) {
+      const int32_t k = top - (int32_t)threadIdx.x;
+      int32_t m = -1;
+      if (k >= 2) {
+        const double r = (double)sqrtf(key_d2(K[k - 1]));
+        if (od_k(k, c.mpart, r) > c.delta2) m = k;
+      }
+      best = max(0, block_max_i32(m, G));
+    }
+    res.count2 = best;
+    if (best > 0) {
+      const double rb = ref_dist(pos, key_idx(K[best - 1]), xcs, c.nx);
+      res.overdens2 = rb > 1e-30 ? od_k(best, c.mpart, rb) : 0.0;
+    }
+  }
+  // profile: particles with r below each profile radius
+  for (int j = threadIdx.x; j < c.nbins; j += GL_T) {
+    const double rj = prof_radius(c, R0, j);
+    int32_t lo = 0, hi = n;  // first rank index with r >= rj
+    while (lo < hi) {
+      const int32_t mid = (lo + hi) >> 1;
+      if ((double)sqrtf(key_d2(K[mid])) < rj) lo = mid + 1;
+      else hi = mid;
+    }
+    out.profile[h * c.nbins + j] = lo;
+  }
+  // boundary particles; centre of mass of the count[0] closest
+  int64_t nbd = 0;
+  double sp[3] = {0, 0, 0}, sv[3] = {0, 0, 0};
+  const int32_t nin = res.count[0];
+  const bool flagv = (c.flags & PSOD_F_VELOCITY) && vel;
+  const double xc[3] = {(double)cr.c[0] + (double)cr.f[0],
+                        (double)cr.c[1] + (double)cr.f[1],
+                        (double)cr.c[2] + (double)cr.f[2]};
+  for (int32_t k = threadIdx.x; k < n; k += GL_T) {
+    const uint64_t key = K[k];
+    const float d2 = key_d2(key);
+    if (d2 < W2 && fabs((double)sqrtf(d2) - R0) <= c.tol_abs) nbd++;
+    if (k < nin) {
+      const int32_t pj = key_idx(key);
+      for (int d = 0; d < 3; d++) {
+        int32_t cj;
+        float fj;
+        split_coord(pos[3 * (int64_t)pj + d], c.nx, &cj, &fj);
+        double dd = ((double)cj + (double)fj) - xc[d];
+        if (dd > 0.5 * c.nx) dd -= c.nx;
+        if (dd < -0.5 * c.nx) dd += c.nx;
+        sp[d] += dd;
+        if (flagv) sv[d] += (double)vel[3 * (int64_t)pj + d];
+      }
+    }
+  }
+  nbd = block_sum_i64(nbd, G);
+  for (int d = 0; d < 3; d++) {
+    const double s = block_sum_f64(sp[d], G);
+    const double v = block_sum_f64(sv[d], G);
+    double cp = (xc[d] + s / (double)nin) / (double)c.nx;
+    cp -= floor(cp);
+    res.com_pos[d] = cp;
+    res.com_vel[d] = v / (double)nin;
+  }
+  res.w = W;
+  res.ngather = n;
+  res.nboundary = (int32_t)nbd;
+  bool acc = (double)res.count[0] * c.mpart >= c.mmin;
+  for (int d = 0; d < 3; d++) acc = acc && xc[d] > 0.0 && xc[d] <= c.nx;
+  res.status = PSOD_H_CLOSED | PSOD_H_LARGE | (acc ? PSOD_H_ACCEPTED : 0);
+  __syncthreads();
+  return 0;
+}
+
 __global__ void __launch_bounds__(GL_T)
 k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
              const float4 *__restrict__ rec, const float *__restrict__ pos,
@@ -2730,7 +2901,8 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
   // same instruction counts, more wave cycles waiting).  The two forms are
   // held to identical results by the tests of the staged growth.
   __shared__ GrowShared G;
-  __shared__ int32_t lh[HIST_LDS];  // inner bins; staging of the scan
+  // inner bins; staging of the scan; the keys of a cube sorted in LDS
+  __shared__ __align__(16) int32_t lh[HIST_LDS];
   // a captured pass whose centre search overflowed: the host runs the pass
   // again without the graph (uniform per workgroup, before any barrier)
   if (abort_flag && *abort_flag) return;
@@ -2782,6 +2954,17 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
         deferred = G.item != 0;
         __syncthreads();
         if (deferred) break;
+      }
+      if (ncube <= c.light_cap) {
+        const int32_t wn = grow_light(c, G, (uint64_t *)lh, cell_start, rec,
+                                      pos, vel, cr, W, (int32_t)ncube, xcs, h,
+                                      out, res);
+        if (wn == 0) {
+          if (threadIdx.x == 0) atomicAdd(&counters[7], 1);
+          break;
+        }
+        W = wn;
+        continue;
       }
       int32_t nb = HIST_MIN;
       while ((int64_t)nb * 4 < ncube && nb < HIST_MAX) nb <<= 1;
@@ -4185,7 +4368,7 @@ static int sod_fof_queue(psod_ctx *ctx, const psod_sod_params *p,
     return PSOD_OK;
   };
   if (!dev)
-    for (int i = 0; i < 12; i++) ctx->sod_stat[i] = 0;
+    for (int i = 0; i < 13; i++) ctx->sod_stat[i] = 0;
   // largest halos first (centre-search tasks and growth), so that the
   // persistent workgroups do not finish on a giant halo picked up last
   auto size_order = [&](int32_t n, const int32_t *dn, int growth) -> int {
@@ -4371,6 +4554,7 @@ static int sod_finish_growth(psod_ctx *ctx, const psod_sod_params *p,
     ctx->sod_stat[0] = nhalo - nlarge;
     ctx->sod_stat[1] = nlarge;
   }
+  ctx->sod_stat[12] = hc[7];  // halos of the first growth launch sorted in LDS
   if (nlarge == 0) return PSOD_OK;
   if (ctx->sod_stat[7] > 0 && !dev) {
     float ms = 0.0f;

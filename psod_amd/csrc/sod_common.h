@@ -25,6 +25,7 @@ struct SodCfg {
   // from which the centre search bounds its boxes first
   int32_t small_max, bound_min;
   int64_t giant_min;  // growth cube particles from which growth is staged
+  int32_t light_cap;  // growth cube particles up to which keys sort in LDS
 };
 
 // overdensity of the k closest particles at radius r, in the reference's

@@ -57,6 +57,41 @@ def test_tunables_do_not_change_results(ctx):
                                    atol=1e-12)
 
 
+def test_growth_cubes_sorted_in_lds_give_the_binned_results(ctx, monkeypatch):
+    """Workgroup halos whose growth cube holds at most PSOD_LIGHT_CAP
+    particles (8192 unless lowered) are grown from their cube's keys sorted
+    in LDS: the same outputs as the binned steps (cap 0), also with the two
+    mixed (cap 1500).  The reach that closes a halo is the path's own."""
+    P, p, pos, vel, lab, ids, nh = _pass(ctx, seed=5)
+    runs = []
+    try:
+        # one-wave halos capped at 256 particles: more workgroup halos
+        ctx.set_tunables(small_max=256)
+        for cap in ("0", "1500", None):
+            if cap is None:
+                monkeypatch.delenv("PSOD_LIGHT_CAP", raising=False)
+            else:
+                monkeypatch.setenv("PSOD_LIGHT_CAP", cap)
+            o = ctx.run_pass(p, pos, lab, nh, vel=vel, ids=ids)
+            runs.append(({k: v.cpu().numpy() for k, v in o.items()},
+                         ctx.sod_stats()))
+    finally:
+        monkeypatch.delenv("PSOD_LIGHT_CAP", raising=False)
+        ctx.set_tunables()
+    o0, s0 = runs[0]
+    assert s0["light"] == 0 and s0["workgroup"] > 0
+    assert 0 < runs[1][1]["light"] < runs[2][1]["light"]
+    assert np.all(o0["status"] & 1)
+    for o, _ in runs[1:]:
+        for k in ("centre", "centre_id", "status", "count", "radius",
+                  "count2", "profile", "com_pos", "nboundary"):
+            np.testing.assert_array_equal(o[k], o0[k], err_msg=k)
+        for k in ("rint", "mint", "overdens", "overdens2"):
+            np.testing.assert_array_equal(o[k], o0[k], err_msg=k)
+        np.testing.assert_allclose(o["com_vel"], o0["com_vel"], rtol=1e-12,
+                                   atol=1e-12)
+
+
 def test_tunables_reject_out_of_range(ctx):
     from psod_amd import PsodError
     with pytest.raises(PsodError):
@@ -118,6 +153,9 @@ def test_staged_growth_of_giant_halos(ctx, monkeypatch):
     p = P.sod_params(nx, N, deltas=(200.0, 500.0), delta2=800.0,
                      velocity=True)
     runs = []
+    # every workgroup halo through the bins (the reach that closes a halo is
+    # compared below, and cubes sorted whole in LDS may close at another)
+    monkeypatch.setenv("PSOD_LIGHT_CAP", "0")
     try:
         for gm, smin in ((1 << 30, None), (1 << 20, None), (4096, None),
                          (4096, "2")):
@@ -130,6 +168,7 @@ def test_staged_growth_of_giant_halos(ctx, monkeypatch):
             assert (st["giants"] > 0) == (gm < 1 << 30), (gm, st)
     finally:
         monkeypatch.delenv("PSOD_GIANT_SORT_MIN", raising=False)
+        monkeypatch.delenv("PSOD_LIGHT_CAP", raising=False)
         ctx.set_tunables()
     o0 = runs[0]
     assert np.all(o0["status"] & 1)
