@@ -215,12 +215,13 @@ k_cic_deposit(const int32_t *__restrict__ cell_start,
   const int32_t nown = opre[TB * TB];
   const int32_t ntot = nown + hpre[NS];
   // register sums of the current lower node and their flush to LDS
-  unsigned long long ra[8];
+  unsigned long long ra[8] = {0ull, 0ull, 0ull, 0ull, 0ull, 0ull, 0ull, 0ull};
   int32_t rid = -1, rix = 0, riy = 0, riz = 0;
-  auto flush = [&]() {
+  // the 4 nodes of x offset h of the current lower node
+  auto flush = [&](int h) {
     if (rid < 0) return;
 #pragma unroll
-    for (int c = 0; c < 8; c++) {
+    for (int c = h; c < 8; c += 2) {
       const int ax = rix + (c & 1), ay = riy + ((c >> 1) & 1), az = riz + (c >> 2);
       if ((unsigned)ax < (unsigned)NN && (unsigned)ay < (unsigned)NN &&
           (unsigned)az < (unsigned)NN)
@@ -281,13 +282,20 @@ k_cic_deposit(const int32_t *__restrict__ cell_start,
                   wz[2] = {1.0f - fz, fz};
       const int32_t id = ((iz + 16) * 64 + (iy + 16)) * 64 + (ix + 16);
       if (id != rid) {
-        flush();
+        // the next cell along x: its lower x plane is the current node's
+        // upper one, whose sums stay in registers
+        const bool carry = rid >= 0 && id == rid + 1;
+        flush(0);
+        if (!carry) flush(1);
+#pragma unroll
+        for (int c = 0; c < 8; c += 2) {
+          ra[c] = carry ? ra[c + 1] : 0ull;
+          ra[c + 1] = 0ull;
+        }
         rid = id;
         rix = ix;
         riy = iy;
         riz = iz;
-#pragma unroll
-        for (int c = 0; c < 8; c++) ra[c] = 0ull;
       }
 #pragma unroll
       for (int c = 0; c < 8; c++)
@@ -295,7 +303,8 @@ k_cic_deposit(const int32_t *__restrict__ cell_start,
                                 FIX_SCALE);
     }
   }
-  flush();
+  flush(0);
+  flush(1);
   __syncthreads();
   const int32_t gx0 = bx0 * 8 * R, gy0 = by0 * 8 * R, gz0 = bz0 * 8 * R;
   for (int i = threadIdx.x; i < NN * NN * NN; i += DEP_T) {
