@@ -14,7 +14,8 @@
 // its 8^3 block, and R*u - floor(R*u) is exact for R in {1, 2}.  Each lane
 // takes K consecutive particles of the sorted list and keeps the 8 node sums
 // in registers while they share the same lower node (dense cells), adding
-// them to LDS only when it changes.  Tiles are dispatched densest first, so
+// them to LDS only when it changes (the upper x plane stays in registers
+// when the next particle lies in the next cell along x).  Tiles are dispatched densest first, so
 // that the long tiles of halo cores overlap the rest.
 #include "psod_internal.h"
 
