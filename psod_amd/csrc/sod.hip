@@ -2178,13 +2178,23 @@ __device__ inline bool grow_reach(const SodCfg &c, const GrowVars &V,
   return true;
 }
 
+// Bins of the radial histogram (uniform in r^3) of a cube of ncube
+// particles: about 4 cube particles per bin up to the bins that LDS holds,
+// at least 16 beyond them, so that fewer particles are counted with global
+// atomics (c3 k_grow_large 14.9 -> 14.0 ms; every bin in LDS: 14.2 ms).
+__device__ inline int32_t hist_bins(int64_t ncube) {
+  int32_t nb = HIST_MIN;
+  while ((int64_t)nb * 4 < ncube && nb < HIST_MAX &&
+         (nb < HIST_LDS || (int64_t)nb * 16 < ncube))
+    nb <<= 1;
+  return nb;
+}
+
 // the radial histogram of a cube of ncube particles and the band of walk 1
 __device__ inline void grow_size(GrowVars &V, int64_t ncube) {
   const float W2 = (float)V.W * (float)V.W;
-  // bins uniform in r^3, about 4 particles of the cube per bin
   V.ncube = ncube;
-  int32_t nb = HIST_MIN;
-  while ((int64_t)nb * 4 < V.ncube && nb < HIST_MAX) nb <<= 1;
+  const int32_t nb = hist_bins(ncube);
   V.nb = nb;
   V.scale = (float)nb / ((float)V.W * (float)V.W * (float)V.W);
   // every particle of the cube is gathered; those of d2 < W2 are binned
@@ -2966,13 +2976,7 @@ k_grow_large(SodCfg c, const int32_t *__restrict__ cell_start,
         W = wn;
         continue;
       }
-      // about 4 cube particles per bin up to the bins that LDS holds, at
-      // least 16 beyond them: fewer bins counted with global atomics (c3
-      // k_grow_large 14.9 -> 14.0 ms; every bin in LDS: 14.2 ms)
-      int32_t nb = HIST_MIN;
-      while ((int64_t)nb * 4 < ncube && nb < HIST_MAX &&
-             (nb < HIST_LDS || (int64_t)nb * 16 < ncube))
-        nb <<= 1;
+      const int32_t nb = hist_bins(ncube);
       const float scale = (float)nb / ((float)W * (float)W * (float)W);
       const int32_t nl = min(nb, HIST_LDS);
       for (int32_t b = threadIdx.x; b < nl; b += GL_T) lh[b] = 0;
