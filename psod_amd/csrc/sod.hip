@@ -1138,7 +1138,7 @@ __device__ __forceinline__ uint64_t pilot_walk(const SodCfg &c,
   return acc;
 }
 
-// Pilot tasks with more than pilot_big records in reach (2^15 unless
+// Pilot tasks with more than pilot_big records in reach (2^14 unless
 // PSOD_PILOT_BIG says otherwise) are split into PILOT_PIECES pieces of their
 // chunks, each summed by its own wave into per-candidate totals (integers:
 // the grouping does not change S).  One wave summing a 5e7-particle halo's
@@ -4029,9 +4029,11 @@ __global__ void k_copy_i32(const int32_t *__restrict__ a,
 }
 
 // records in reach above which a pilot task is split; PSOD_PILOT_BIG
-// overrides the default for tests (at least 64)
+// overrides the default for tests (at least 64).  Pilots plus split pieces
+// at c3 / c4 (ms): 2^13 4.1 / 2.2, 2^14 3.0 / 2.3, 2^15 3.8 / 2.6, 2^16
+// 5.0 / 2.8.
 static int32_t pilot_big() {
-  int32_t v = 1 << 15;
+  int32_t v = 1 << 14;
   if (const char *e = getenv("PSOD_PILOT_BIG")) {
     const long long x = atoll(e);
     if (x >= 64 && x <= INT32_MAX) v = (int32_t)x;
