@@ -296,7 +296,11 @@ __device__ void small_growth_and_output(const SodCfg &c, SmallLds &S,
                                         HaloPlan *plan, int32_t *large,
                                         int32_t *counters);
 
-__global__ void __launch_bounds__(64)
+// At most 176 vector registers as allocated (blocks of 8): two of these
+// waves then leave room for two waves of the centre-search bound on the
+// same SIMD.  Its key partition pushed the allocation to 184 (bound 63.4 ->
+// 67.4 ms beside it at 1024^3); held to 176, the bound took 62.0 ms.
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_num_vgpr(168)))
 k_sod_small(SodCfg c, const int32_t *__restrict__ cell_start,
             const float4 *__restrict__ rec, const float *__restrict__ pos,
             const float *__restrict__ vel, const int64_t *__restrict__ ids,
@@ -471,21 +475,59 @@ __device__ void small_growth_and_output(const SodCfg &c, SmallLds &S,
       W = WMAX_S + 1;  // defer
       continue;
     }
-    for (int32_t j = l; j < ncube; j += 64) {
-      const int q = bsearch_last_le(S.b.pre, ncell, j);
-      const float4 r = rec[S.b.st[q] + (j - S.b.pre[q])];
-      int di, dj, dk;
-      walk_off(q, W, &di, &dj, &dk);
-      int32_t lc[3];
-      key_at(c.nx, cr.c, di, dj, dk, lc);
-      const float dx = rel1(r.x, lc[0], di, cr.f[0]);
-      const float dy = rel1(r.y, lc[1], dj, cr.f[1]);
-      const float dz = rel1(r.z, lc[2], dk, cr.f[2]);
-      const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-      S.b.key[j] = d2key(d2, __float_as_int(r.w));
+    // Keys of r < W (the only ones that can be ranked before a crossing)
+    // go to the front, the others to the back of the key array; only the
+    // front is sorted when the back leaves room for its padding (the cube's
+    // corners hold half of its particles or more).  Otherwise the back
+    // moves down behind the front and every key is sorted.
+    const double Wd = (double)W;
+    int32_t ninr = 0, nfar = 0;
+    const uint64_t below = l == 0 ? 0ull : (~0ull >> (64 - l));
+    for (int32_t base = 0; base < ncube; base += 64) {
+      const int32_t j = base + l;
+      uint64_t key = 0;
+      bool inner = false;
+      if (j < ncube) {
+        const int q = bsearch_last_le(S.b.pre, ncell, j);
+        const float4 r = rec[S.b.st[q] + (j - S.b.pre[q])];
+        int di, dj, dk;
+        walk_off(q, W, &di, &dj, &dk);
+        int32_t lc[3];
+        key_at(c.nx, cr.c, di, dj, dk, lc);
+        const float dx = rel1(r.x, lc[0], di, cr.f[0]);
+        const float dy = rel1(r.y, lc[1], dj, cr.f[1]);
+        const float dz = rel1(r.z, lc[2], dk, cr.f[2]);
+        const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        key = d2key(d2, __float_as_int(r.w));
+        inner = (double)sqrtf(d2) < Wd;
+      }
+      const uint64_t mi = ballot64(j < ncube && inner);
+      const uint64_t mf = ballot64(j < ncube && !inner);
+      if (j < ncube) {
+        if (inner) S.b.key[ninr + __popcll(mi & below)] = key;
+        else S.b.key[CAPG - 1 - (nfar + __popcll(mf & below))] = key;
+      }
+      ninr += __popcll(mi);
+      nfar += __popcll(mf);
     }
-    const int npad = next_pow2(ncube < 64 ? 64 : ncube);
-    for (int32_t j = ncube + l; j < npad; j += 64) S.b.key[j] = ~0ull;
+    __syncthreads();
+    int32_t npad = next_pow2(ninr < 64 ? 64 : ninr);
+    const bool part = npad + nfar <= CAPG;  // the far keys stay at the back
+    if (!part) {
+      // the far keys down to [ninr, ncube), in ascending chunks (each read
+      // before it is written; destinations lie below their sources)
+      for (int32_t b0 = 0; b0 < nfar; b0 += 64) {
+        const int32_t t = b0 + l;
+        uint64_t v = 0;
+        if (t < nfar) v = S.b.key[CAPG - nfar + t];
+        __syncthreads();
+        if (t < nfar) S.b.key[ninr + t] = v;
+        __syncthreads();
+      }
+      npad = next_pow2(ncube < 64 ? 64 : ncube);
+    }
+    const int32_t nsort = part ? ninr : ncube;  // keys in sorted order
+    for (int32_t j = nsort + l; j < npad; j += 64) S.b.key[j] = ~0ull;
     __syncthreads();
     lds_bitonic<64>(S.b.key, npad);
     // ---- first crossing of every threshold -----------------------------
@@ -497,10 +539,12 @@ __device__ void small_growth_and_output(const SodCfg &c, SmallLds &S,
            base += 64) {
         const int64_t k = base + l;
         bool hit = false, over = false;
-        if (k <= ncube) {
+        if (k <= nsort) {
           const double r = (double)sqrtf(key_d2(S.b.key[k - 1]));
-          over = r >= (double)W;
+          over = r >= Wd;
           hit = over || od_k(k, c.mpart, r) <= c.delta[di];
+        } else if (k <= ncube) {
+          hit = over = true;  // a far key: r >= W
         }
         const uint64_t m = ballot64(hit);
         if (m) {
@@ -561,12 +605,15 @@ __device__ void small_growth_and_output(const SodCfg &c, SmallLds &S,
     // ---- profile --------------------------------------------------------
     for (int j = l; j < c.nbins; j += 64) {
       const double rj = prof_radius(c, R0, j);
-      int lo = 0, hi = ncube;  // first index with r >= rj
+      int lo = 0, hi = nsort;  // first index with r >= rj
       while (lo < hi) {
         const int mid = (lo + hi) >> 1;
         if ((double)sqrtf(key_d2(S.b.key[mid])) < rj) lo = mid + 1;
         else hi = mid;
       }
+      if (part && rj > Wd)  // far keys (r >= W) below a radius beyond W
+        for (int32_t t = CAPG - nfar; t < CAPG; t++)
+          if ((double)sqrtf(key_d2(S.b.key[t])) < rj) lo++;
       out.profile[h * c.nbins + j] = lo;
     }
     // ---- boundary particles, centre of mass of the count[0] closest -------
@@ -578,7 +625,9 @@ __device__ void small_growth_and_output(const SodCfg &c, SmallLds &S,
                           (double)cr.c[1] + (double)cr.f[1],
                           (double)cr.c[2] + (double)cr.f[2]};
     for (int32_t k = l; k < ncube; k += 64) {
-      const uint64_t key = S.b.key[k];
+      // rank order for the sorted keys, then the far keys at the back
+      const uint64_t key =
+          S.b.key[k < nsort ? k : CAPG - nfar + (k - nsort)];
       const double r = (double)sqrtf(key_d2(key));
       if (fabs(r - R0) <= c.tol_abs) nbd++;
       if (k < nin) {
